@@ -1,0 +1,57 @@
+// Packed mechanism tables as the HIP kernels read them (host packs, device stages into LDS).
+//
+// One reaction record is 12 doubles + 4 ints; one species record is 17 doubles. The host
+// builds the blob once per reactor from the generated pr_mech_t tables (pr_create) and the
+// kernels copy it into LDS at launch, so every Arrhenius / third-body / falloff parameter the
+// rate and Jacobian code touches is an LDS read.
+#ifndef PR_DEVICE_MECH_H
+#define PR_DEVICE_MECH_H
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "mech_common.h"
+
+#define PR_AL16(x) (((x) + 15) & ~15)
+
+namespace prk {
+
+struct RxRec {
+  double lnA, beta, Ta;       // k_inf (or k) = exp(lnA + beta ln T - Ta / T)
+  double lnA0, beta0, Ta0;    // low-pressure limit (falloff)
+  double troe[4];             // a, T***, T*, T**
+  double effd;                // default collision efficiency
+  double pad;
+  int32_t info;               // type | rev << 4 | nreac << 8 | nprod << 12 | ntroe << 16
+  int32_t reac;               // three 8-bit species slots
+  int32_t prod;               // three 8-bit species slots
+  int32_t eff;                // eff_begin | eff_end << 16
+};
+
+struct SpRec {
+  double nasa[15];            // T_mid, high a1..a7, low a1..a7
+  double W, invW;
+};
+
+// Blob layout (byte offsets, 16-byte aligned sections):
+//   SpRec sp[nsp] | RxRec rx[nr] | int eff_sp[neff] | double eff_val[neff]
+//   | int inc_start[nsp+1] | int inc_rnu[ninc]  (reaction | nu << 16, nu as signed 16-bit)
+struct BlobLayout {
+  int nsp, nr, neff, ninc;
+  int off_sp, off_rx, off_effsp, off_effval, off_incs, off_inc, bytes;
+};
+
+__host__ __device__ inline BlobLayout blob_layout(int nsp, int nr, int neff, int ninc) {
+  BlobLayout L;
+  L.nsp = nsp; L.nr = nr; L.neff = neff; L.ninc = ninc;
+  int o = 0;
+  L.off_sp = o;     o = PR_AL16(o + nsp * (int)sizeof(SpRec));
+  L.off_rx = o;     o = PR_AL16(o + nr * (int)sizeof(RxRec));
+  L.off_effsp = o;  o = PR_AL16(o + (neff > 0 ? neff : 1) * 4);
+  L.off_effval = o; o = PR_AL16(o + (neff > 0 ? neff : 1) * 8);
+  L.off_incs = o;   o = PR_AL16(o + (nsp + 1) * 4);
+  L.off_inc = o;    o = PR_AL16(o + (ninc > 0 ? ninc : 1) * 4);
+  L.bytes = o;
+  return L;
+}
+
+}  // namespace prk
+#endif

@@ -1,0 +1,750 @@
+// The persistent react kernel: one BDF state machine per group of G lanes, one cell at a time,
+// cells pulled from an atomic work queue. Semantics follow CVODE's CV_BDF / Newton / dense
+// direct path as PelePhysics' ReactorCvode uses it (called from
+// Source/PeleLMeX_Reactions.cpp:101-108), integrated to tout = dt_react and interpolated back
+// from the first step that passes it.
+//
+// Register budget: the vectors a step touches every iteration (Nordsieck history, weights,
+// corrections, LU rows) live in VGPRs; the group's scalar control state (step size, order,
+// counters, method coefficients) lives in a per-group LDS control block, re-read after each
+// right-hand side, so it does not occupy registers across the expensive phases.
+#ifndef PR_REACT_KERNEL_H
+#define PR_REACT_KERNEL_H
+#include <float.h>
+#include "bdf_kernel.h"
+
+namespace prk {
+
+enum { PH_FETCH = 0, PH_F0, PH_HIN, PH_NEWTON, PH_RESTART };
+enum { NF_FIRST = 0, NF_CONV = 1, NF_ERR = 2 };
+enum { CF_NONE = 0, CF_BADJ = 1, CF_OTHER = 2 };
+
+// Per-group control block (LDS). All lanes of a group hold and write identical values.
+struct Ctl {
+  double tn, h, hscale, hprime, eta, etamax, rl1, gamma, gammap, gamrat, crate, delp, acnrm;
+  double saved_tq5, saved_t, t_eval, hlb, hub, hg, rhoh0, rhohsrc;
+  double tau[QMAXC + 1], tq[6], l[QMAXC + 1];
+  int q, qprime, L, qwait, nscon, ncf, nef, nflag, convfail, mnewt, jcur, jbad, callSetup, count1;
+  int phase, cell, nst, nstlp, nstlj, nstloc, nfe, nje, nset, nni, netf, ncfn, guard;
+  unsigned int s_cells, s_nst, s_nfe, s_nje, s_nset, s_nni, s_netf, s_ncfn, s_fail;
+};
+constexpr int CTL_DOUBLES = (int)((sizeof(Ctl) + 7) / 8);
+
+__device__ __forceinline__ long aoff(const Arr& a, int i, int j, int k) {
+  return (long)(i - a.bx) + (long)(j - a.by) * a.js + (long)(k - a.bz) * a.ks;
+}
+
+__device__ __forceinline__ void clobber() { asm volatile("" ::: "memory"); }
+
+template <int N, int R>
+constexpr int group_doubles() {
+  return Smem<N, R>::scratch_doubles + CTL_DOUBLES;
+}
+
+template <int N, int R, int G>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) react_kernel(KParams P) {
+  constexpr int n = N + 1;
+  constexpr int RPL = Cell<N, R, G>::RPL;
+  constexpr int Q = QMAXC;
+  extern __shared__ __align__(16) unsigned char smem[];
+  {
+    const uint4* srcb = reinterpret_cast<const uint4*>(P.blob);
+    uint4* dst = reinterpret_cast<uint4*>(smem);
+    for (int b = threadIdx.x; b < P.blob_bytes / 16; b += blockDim.x) dst[b] = srcb[b];
+  }
+  __syncthreads();
+  const BlobLayout BL = blob_layout(N, R, P.neff, P.ninc);
+  Smem<N, R> S;
+  S.sp = reinterpret_cast<const SpRec*>(smem + BL.off_sp);
+  S.rx = reinterpret_cast<const RxRec*>(smem + BL.off_rx);
+  S.effsp = reinterpret_cast<const int*>(smem + BL.off_effsp);
+  S.effval = reinterpret_cast<const double*>(smem + BL.off_effval);
+  S.incs = reinterpret_cast<const int*>(smem + BL.off_incs);
+  S.inc = reinterpret_cast<const int*>(smem + BL.off_inc);
+  const int lane = threadIdx.x % G;
+  const int grp = threadIdx.x / G;
+  double* scr = reinterpret_cast<double*>(smem + BL.bytes) + (long)grp * group_doubles<N, R>();
+  Ctl& c = *reinterpret_cast<Ctl*>(scr);
+  S.Js = scr + CTL_DOUBLES;
+  S.sC = S.Js + n * n;
+  S.sG = S.sC + N;
+  S.sH = S.sG + N;
+  S.sHm = S.sH + N;
+  S.sCpm = S.sHm + N;
+  S.rec = S.sCpm + N;
+  S.perm = reinterpret_cast<int*>(S.rec + 5 * R);
+
+  const double tout = P.dt;
+  const int qmax = P.max_order < 1 ? 1 : (P.max_order > Q ? Q : P.max_order);
+
+  c.s_cells = c.s_nst = c.s_nfe = c.s_nje = c.s_nset = c.s_nni = c.s_netf = c.s_ncfn = c.s_fail = 0u;
+  c.phase = PH_FETCH;
+
+  // vectors kept in registers (rows lane, lane + G, ...)
+  double zn[Q + 1][RPL], ewt[RPL], acor[RPL], yv[RPL], f[RPL], src[RPL], atol_r[RPL];
+  double lu[RPL][n];
+  int pst[RPL];
+  double rinv[RPL];
+#pragma unroll
+  for (int s = 0; s < RPL; ++s) {
+    int i = lane + G * s;
+    atol_r[s] = (P.atolv && i < n) ? P.atolv[i] : P.atol;
+  }
+
+  auto cell_ijk = [&](int cl, int& i, int& j, int& k) __attribute__((always_inline)) {
+    const unsigned u = (unsigned)cl;
+    i = P.lo0 + (int)(u % (unsigned)P.nx);
+    j = P.lo1 + (int)((u / (unsigned)P.nx) % (unsigned)P.ny);
+    k = P.lo2 + (int)(u / ((unsigned)P.nx * (unsigned)P.ny));
+  };
+
+  auto finish = [&](int code, const double (&out)[RPL]) __attribute__((always_inline)) {
+    int i, j, k;
+    cell_ijk(c.cell, i, j, k);
+    const long offY = aoff(P.rY, i, j, k), offT = aoff(P.T, i, j, k);
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      int ii = lane + G * s;
+      if (ii < N) P.rY.p[offY + (long)ii * P.rY.ns] = out[s];
+      else if (ii == N) P.T.p[offT] = out[s];
+    }
+    if (lane == 0) {
+      P.rH.p[aoff(P.rH, i, j, k)] = c.rhoh0 + tout * c.rhohsrc;
+      P.FC.p[aoff(P.FC, i, j, k)] = (double)c.nfe;
+      if (P.status) P.status[c.cell] = code;
+    }
+    c.s_cells += 1u;
+    c.s_nst += c.nst; c.s_nfe += c.nfe; c.s_nje += c.nje; c.s_nset += c.nset; c.s_nni += c.nni;
+    c.s_netf += c.netf; c.s_ncfn += c.ncfn;
+    if (code != ST_OK) c.s_fail += 1u;
+    c.phase = PH_FETCH;
+  };
+
+  auto set_ewt = [&](const double (&y)[RPL]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) ewt[s] = 1.0 / (P.rtol * fabs(y[s]) + atol_r[s]);
+  };
+
+  auto rescale = [&]() __attribute__((always_inline)) {
+    const double eta = c.eta;
+    const int q = c.q;
+    double factor = eta;
+#pragma unroll
+    for (int j = 1; j <= Q; ++j) {
+      if (j <= q) {
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) zn[j][s] *= factor;
+        factor *= eta;
+      }
+    }
+    c.h = c.hscale * eta;
+    c.hscale = c.h;
+    c.nscon = 0;
+  };
+
+  auto predict = [&]() __attribute__((always_inline)) {
+    c.tn += c.h;
+    const int q = c.q;
+#pragma unroll
+    for (int k = 1; k <= Q; ++k)
+#pragma unroll
+      for (int j = Q; j >= k; --j)
+        if (j <= q)
+#pragma unroll
+          for (int s = 0; s < RPL; ++s) zn[j - 1][s] += zn[j][s];
+  };
+
+  auto restore = [&]() __attribute__((always_inline)) {
+    c.tn = c.saved_t;
+    const int q = c.q;
+#pragma unroll
+    for (int k = 1; k <= Q; ++k)
+#pragma unroll
+      for (int j = Q; j >= k; --j)
+        if (j <= q)
+#pragma unroll
+          for (int s = 0; s < RPL; ++s) zn[j - 1][s] -= zn[j][s];
+  };
+
+  // l (method coefficients), tq (error constants), gamma for the current q, h, tau history
+  auto set_bdf = [&]() __attribute__((always_inline)) {
+    const int q = c.q;
+    const double h = c.h;
+    double* l = c.l;
+    double alpha0 = -1.0, alpha0_hat = -1.0, xi_inv = 1.0, xistar_inv = 1.0, hsum = h;
+    l[0] = 1.0;
+    l[1] = 1.0;
+    for (int i = 2; i <= Q; ++i) l[i] = 0.0;
+    if (q > 1) {
+      for (int j = 2; j < q; ++j) {
+        hsum += c.tau[j - 1];
+        xi_inv = h / hsum;
+        alpha0 -= 1.0 / j;
+        for (int i = j; i >= 1; --i) l[i] += l[i - 1] * xi_inv;
+      }
+      alpha0 -= 1.0 / q;
+      xistar_inv = -l[1] - alpha0;
+      hsum += c.tau[q - 1];
+      xi_inv = h / hsum;
+      alpha0_hat = -l[1] - xi_inv;
+      for (int i = q; i >= 1; --i) l[i] += l[i - 1] * xistar_inv;
+    }
+    const double A1 = 1.0 - alpha0_hat + alpha0;
+    const double A2 = 1.0 + q * A1;
+    const double lq = l[q];
+    c.tq[2] = fabs(A1 / (alpha0 * A2));
+    c.tq[5] = fabs(A2 * xistar_inv / (lq * xi_inv));
+    if (c.qwait == 1) {
+      if (q > 1) {
+        double C = xistar_inv / lq;
+        double A3 = alpha0 + 1.0 / q;
+        double A4 = alpha0_hat + xi_inv;
+        double Cpinv = (1.0 - A4 + A3) / A3;
+        c.tq[1] = fabs(C * Cpinv);
+      } else {
+        c.tq[1] = 1.0;
+      }
+      hsum += c.tau[q];
+      xi_inv = h / hsum;
+      double A5 = alpha0 - 1.0 / (q + 1);
+      double A6 = alpha0_hat - xi_inv;
+      double Cppinv = (1.0 - A6 + A5) / A2;
+      c.tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
+    }
+    c.tq[4] = 0.1 / c.tq[2];
+    c.rl1 = 1.0 / l[1];
+    c.gamma = h * c.rl1;
+    if (c.nst == 0) c.gammap = c.gamma;
+    c.gamrat = (c.nst > 0) ? c.gamma / c.gammap : 1.0;
+  };
+
+  auto decrease_bdf = [&]() __attribute__((always_inline)) {
+    const int q = c.q;
+    double* l = c.l;
+    for (int i = 0; i <= Q; ++i) l[i] = 0.0;
+    l[2] = 1.0;
+    double hsum = 0.0;
+    for (int j = 1; j <= q - 2; ++j) {
+      hsum += c.tau[j];
+      double xi = hsum / c.hscale;
+      for (int i = j + 2; i >= 2; --i) l[i] = l[i] * xi + l[i - 1];
+    }
+    double zq[RPL];
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) zq[s] = 0.0;
+#pragma unroll
+    for (int j = 0; j <= Q; ++j)
+      if (j == q)
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) zq[s] = zn[j][s];
+#pragma unroll
+    for (int j = 2; j < Q; ++j)
+      if (j < q) {
+        const double lj = l[j];
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) zn[j][s] -= lj * zq[s];
+      }
+  };
+
+  auto increase_bdf = [&]() __attribute__((always_inline)) {
+    const int q = c.q, L = c.L;
+    double* l = c.l;
+    for (int i = 0; i <= Q; ++i) l[i] = 0.0;
+    double alpha1 = 1.0, prod = 1.0, xiold = 1.0, alpha0 = -1.0, hsum = c.hscale;
+    l[2] = 1.0;
+    if (q > 1) {
+      for (int j = 1; j < q; ++j) {
+        hsum += c.tau[j + 1];
+        double xi = hsum / c.hscale;
+        prod *= xi;
+        alpha0 -= 1.0 / (j + 1);
+        alpha1 += 1.0 / xi;
+        for (int i = j + 2; i >= 2; --i) l[i] = l[i] * xiold + l[i - 1];
+        xiold = xi;
+      }
+    }
+    const double A1 = (-alpha0 - alpha1) / prod;
+    double zL[RPL], zmax[RPL];
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) zmax[s] = 0.0;
+#pragma unroll
+    for (int j = 0; j <= Q; ++j)
+      if (j == qmax)
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) zmax[s] = zn[j][s];
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) zL[s] = A1 * zmax[s];
+#pragma unroll
+    for (int j = 0; j <= Q; ++j)
+      if (j == L)
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) zn[j][s] = zL[s];
+#pragma unroll
+    for (int j = 2; j <= Q; ++j)
+      if (j <= q) {
+        const double lj = l[j];
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) zn[j][s] += lj * zL[s];
+      }
+  };
+
+  auto adjust_params = [&]() __attribute__((always_inline)) {
+    if (c.qprime != c.q) {
+      const int dq = c.qprime - c.q;
+      if (!(c.q == 2 && dq != 1)) {
+        if (dq == 1) increase_bdf();
+        else if (dq == -1) decrease_bdf();
+      }
+      c.q = c.qprime;
+      c.L = c.q + 1;
+      c.qwait = c.L;
+    }
+    rescale();
+  };
+
+  auto attempt = [&]() __attribute__((always_inline)) {
+    predict();
+    set_bdf();
+    const int nflag = c.nflag;
+    c.convfail = (nflag == NF_FIRST || nflag == NF_ERR) ? CF_NONE : CF_OTHER;
+    c.callSetup = (nflag == NF_CONV) || (nflag == NF_ERR) || (c.nst == 0) || (c.nst >= c.nstlp + 20) ||
+                  (fabs(c.gamrat - 1.0) > 0.3);
+    c.jbad = 0;
+    c.mnewt = 0;
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      acor[s] = 0.0;
+      yv[s] = zn[0][s];
+    }
+    c.t_eval = c.tn;
+    c.phase = PH_NEWTON;
+  };
+
+  auto begin_step = [&]() __attribute__((always_inline)) {
+    if (c.nst > 0) set_ewt(zn[0]);
+    if (c.nstloc >= P.max_steps) {
+      finish(ST_TOO_MUCH_WORK, zn[0]);
+      return;
+    }
+    if (c.nst > 0 && c.hprime != c.h) adjust_params();
+    c.saved_t = c.tn;
+    c.ncf = 0;
+    c.nef = 0;
+    c.nflag = NF_FIRST;
+    attempt();
+  };
+
+  auto start_integration = [&]() __attribute__((always_inline)) {
+    const double h = c.h;
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) zn[1][s] *= h;
+    c.hscale = h;
+    c.hprime = h;
+    c.nstloc = 0;
+    begin_step();
+  };
+
+  const int guard_max = 20 * P.max_steps + 1000;
+
+  for (;;) {
+    clobber();
+    if (c.phase == PH_FETCH) {
+      unsigned int cl = 0;
+      if (lane == 0) cl = atomicAdd(P.queue, 1u);
+      cl = __shfl(cl, 0, G);
+      if ((long)cl >= P.ncells) break;
+      int i, j, k;
+      cell_ijk((int)cl, i, j, k);
+      c.cell = (int)cl;
+      if (P.has_mask) {
+        long mo = (long)(i - P.mask.bx) + (long)(j - P.mask.by) * P.mask.js + (long)(k - P.mask.bz) * P.mask.ks;
+        if (P.mask.p[mo] == -1) {
+          if (lane == 0) {
+            P.FC.p[aoff(P.FC, i, j, k)] = 0.0;
+            if (P.status) P.status[cl] = ST_OK;
+          }
+          continue;
+        }
+      }
+      const long offY = aoff(P.rY, i, j, k), offS = aoff(P.rYsrc, i, j, k);
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) {
+        int ii = lane + G * s;
+        double y0 = 0.0;
+        src[s] = 0.0;
+        if (ii < N) {
+          y0 = P.rY.p[offY + (long)ii * P.rY.ns];
+          src[s] = P.rYsrc.p[offS + (long)ii * P.rYsrc.ns];
+        } else if (ii == N) {
+          y0 = P.T.p[aoff(P.T, i, j, k)];
+        }
+        zn[0][s] = y0;
+        yv[s] = y0;
+#pragma unroll
+        for (int j2 = 1; j2 <= Q; ++j2) zn[j2][s] = 0.0;
+      }
+      c.rhoh0 = P.rH.p[aoff(P.rH, i, j, k)];
+      c.rhohsrc = P.rHsrc.p[aoff(P.rHsrc, i, j, k)];
+      c.nfe = c.nje = c.nset = c.nni = c.netf = c.ncfn = 0;
+      c.nst = c.nstlp = c.nstlj = 0;
+      c.q = 1; c.L = 2; c.qwait = 2; c.qprime = 1;
+      c.etamax = 10000.0; c.nscon = 0; c.saved_tq5 = 0.0; c.crate = 1.0; c.delp = 0.0;
+      c.gamrat = 1.0; c.gammap = 1.0; c.tn = 0.0; c.jcur = 0; c.eta = 1.0;
+      for (int j2 = 0; j2 <= Q; ++j2) c.tau[j2] = 0.0;
+      for (int j2 = 0; j2 < 6; ++j2) c.tq[j2] = 0.0;
+      set_ewt(zn[0]);
+      c.t_eval = 0.0;
+      c.guard = 0;
+      c.phase = PH_F0;
+    }
+    if (++c.guard > guard_max) {
+      finish(ST_GUARD, zn[0]);
+      continue;
+    }
+
+    rhs_eval<N, R, G>(S, lane, c.t_eval, yv, src, c.rhoh0, c.rhohsrc, f);
+    clobber();
+    c.nfe += 1;
+    const int phase = c.phase;
+
+    if (phase == PH_F0) {
+      double hinv = 0.0;
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) {
+        zn[1][s] = f[s];
+        int i = lane + G * s;
+        if (i < n) {
+          double r = fabs(f[s]) / (0.1 * fabs(zn[0][s]) + atol_r[s]);
+          hinv = r > hinv ? r : hinv;
+        }
+      }
+#pragma unroll
+      for (int m = G / 2; m >= 1; m >>= 1) hinv = fmax(hinv, __shfl_xor(hinv, m, G));
+      const double tround = DBL_EPSILON * fabs(tout);
+      const double hlb = 100.0 * tround;
+      double hub = 0.1 * fabs(tout);
+      if (hub * hinv > 1.0) hub = 1.0 / hinv;
+      const double hg = sqrt(hlb * hub);
+      c.hlb = hlb;
+      c.hub = hub;
+      c.hg = hg;
+      if (hub < hlb) {
+        c.h = hg;
+        start_integration();
+      } else {
+        c.count1 = 1;
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) yv[s] = zn[0][s] + hg * zn[1][s];
+        c.t_eval = hg;
+        c.phase = PH_HIN;
+      }
+    } else if (phase == PH_HIN) {
+      const double hg = c.hg, hub = c.hub;
+      double tmp[RPL];
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) tmp[s] = (f[s] - zn[1][s]) / hg;
+      const double yddnrm = wrms<N, R, G>(lane, tmp, ewt);
+      double hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
+      bool done = false;
+      if (c.count1 == 4) {
+        done = true;
+      } else {
+        double hrat = hnew / hg;
+        if (hrat > 0.5 && hrat < 2.0) done = true;
+        else if (c.count1 > 1 && hrat > 2.0) {
+          hnew = hg;
+          done = true;
+        }
+      }
+      if (done) {
+        double h0 = 0.5 * hnew;
+        if (h0 < c.hlb) h0 = c.hlb;
+        if (h0 > hub) h0 = hub;
+        c.h = h0;
+        start_integration();
+      } else {
+        c.hg = hnew;
+        c.count1 += 1;
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) yv[s] = zn[0][s] + hnew * zn[1][s];
+        c.t_eval = hnew;
+      }
+    } else if (phase == PH_RESTART) {
+      const double h = c.h;
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) zn[1][s] = h * f[s];
+      attempt();
+    } else {  // PH_NEWTON: f = f(tn, zn0 + acor)
+      double res[RPL];
+      {
+        const double rl1 = c.rl1, gamma = c.gamma;
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) res[s] = rl1 * zn[1][s] + acor[s] - gamma * f[s];
+      }
+      bool singular = false;
+      if (c.callSetup) {
+        const int cf = c.jbad ? CF_BADJ : c.convfail;
+        const double dgamma = fabs(c.gamma / c.gammap - 1.0);
+        const bool jnew = (c.nst == 0) || (c.nst >= c.nstlj + 51) || (cf == CF_BADJ && dgamma < 0.2) || (cf == CF_OTHER);
+        if (jnew) {
+          c.nje += 1;
+          c.nstlj = c.nst;
+          c.jcur = 1;
+          jac_eval<N, R, G>(S, lane, yv, src, c.rhohsrc);
+          clobber();
+        } else {
+          c.jcur = 0;
+        }
+        singular = lu_factor<N, R, G>(S, lane, c.gamma, lu, pst, rinv);
+        clobber();
+        c.nset += 1;
+        c.gamrat = 1.0;
+        c.gammap = c.gamma;
+        c.crate = 1.0;
+        c.nstlp = c.nst;
+        c.callSetup = 0;
+      }
+      int outcome = 0;  // 0 continue, 1 converged, 2 failed
+      if (!singular) {
+        c.nni += 1;
+        double b[RPL], dlt[RPL];
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) b[s] = -res[s];
+        lu_solve<N, R, G>(S, lane, lu, pst, rinv, b, dlt);
+        clobber();
+        const double gamrat = c.gamrat;
+        if (gamrat != 1.0) {
+          const double sc = 2.0 / (1.0 + gamrat);
+#pragma unroll
+          for (int s = 0; s < RPL; ++s) dlt[s] *= sc;
+        }
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) acor[s] += dlt[s];
+        const double del = wrms<N, R, G>(lane, dlt, ewt);
+        const int mnewt = c.mnewt;
+        if (mnewt > 0) c.crate = fmax(0.3 * c.crate, del / c.delp);
+        const double dcon = del * fmin(1.0, c.crate) / c.tq[4];
+        if (dcon <= 1.0) {
+          c.acnrm = (mnewt == 0) ? del : wrms<N, R, G>(lane, acor, ewt);
+          c.jcur = 0;
+          outcome = 1;
+        } else if (mnewt >= 1 && del > 2.0 * c.delp) {
+          outcome = 2;
+        } else {
+          c.delp = del;
+          c.mnewt = mnewt + 1;
+          if (mnewt + 1 >= 3) outcome = 2;
+        }
+      } else {
+        outcome = 2;
+      }
+      if (outcome == 0) {
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) yv[s] = zn[0][s] + acor[s];
+      } else if (outcome == 2) {
+        if (!c.jcur) {
+          c.callSetup = 1;
+          c.jbad = 1;
+          c.mnewt = 0;
+#pragma unroll
+          for (int s = 0; s < RPL; ++s) {
+            acor[s] = 0.0;
+            yv[s] = zn[0][s];
+          }
+        } else {
+          c.ncfn += 1;
+          restore();
+          c.ncf += 1;
+          c.etamax = 1.0;
+          if (c.ncf == 10) {
+            finish(ST_CONV_FAILURE, zn[0]);
+          } else {
+            c.eta = 0.25;
+            c.nflag = NF_CONV;
+            rescale();
+            attempt();
+          }
+        }
+      } else {
+        const double dsm = c.acnrm * c.tq[2];
+        if (dsm <= 1.0) {
+          // complete the step
+          c.nst += 1;
+          c.nscon += 1;
+          const int q = c.q;
+          for (int i = q; i >= 2; --i) c.tau[i] = c.tau[i - 1];
+          if (q == 1 && c.nst > 1) c.tau[2] = c.tau[1];
+          c.tau[1] = c.h;
+#pragma unroll
+          for (int j = 0; j <= Q; ++j)
+            if (j <= q) {
+              const double lj = c.l[j];
+#pragma unroll
+              for (int s = 0; s < RPL; ++s) zn[j][s] += lj * acor[s];
+            }
+          c.qwait -= 1;
+          if (c.qwait == 1 && q != qmax) {
+#pragma unroll
+            for (int j = 0; j <= Q; ++j)
+              if (j == qmax)
+#pragma unroll
+                for (int s = 0; s < RPL; ++s) zn[j][s] = acor[s];
+            c.saved_tq5 = c.tq[5];
+          }
+          // choose the next step size and order
+          if (c.etamax == 1.0) {
+            if (c.qwait < 2) c.qwait = 2;
+            c.qprime = q;
+            c.hprime = c.h;
+            c.eta = 1.0;
+          } else {
+            const int L = c.L;
+            const double etaq = 1.0 / (pow(6.0 * dsm, 1.0 / L) + 1e-6);
+            if (c.qwait != 0) {
+              c.eta = etaq;
+              c.qprime = q;
+            } else {
+              c.qwait = 2;
+              double etaqm1 = 0.0, etaqp1 = 0.0;
+              if (q > 1) {
+                double zq[RPL];
+#pragma unroll
+                for (int s = 0; s < RPL; ++s) zq[s] = 0.0;
+#pragma unroll
+                for (int j = 0; j <= Q; ++j)
+                  if (j == q)
+#pragma unroll
+                    for (int s = 0; s < RPL; ++s) zq[s] = zn[j][s];
+                const double ddn = wrms<N, R, G>(lane, zq, ewt) * c.tq[1];
+                etaqm1 = 1.0 / (pow(6.0 * ddn, 1.0 / q) + 1e-6);
+              }
+              if (q != qmax && c.saved_tq5 != 0.0) {
+                const double cquot = (c.tq[5] / c.saved_tq5) * pow(c.h / c.tau[2], (double)L);
+                double tmp[RPL];
+#pragma unroll
+                for (int s = 0; s < RPL; ++s) tmp[s] = acor[s];
+#pragma unroll
+                for (int j = 0; j <= Q; ++j)
+                  if (j == qmax)
+#pragma unroll
+                    for (int s = 0; s < RPL; ++s) tmp[s] = acor[s] - cquot * zn[j][s];
+                const double dup = wrms<N, R, G>(lane, tmp, ewt) * c.tq[3];
+                etaqp1 = 1.0 / (pow(10.0 * dup, 1.0 / (L + 1)) + 1e-6);
+              }
+              const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
+              if (etam < 1.5) {
+                c.eta = 1.0;
+                c.qprime = q;
+              } else if (etam == etaqm1) {
+                c.eta = etaqm1;
+                c.qprime = q - 1;
+              } else if (etam == etaq) {
+                c.eta = etaq;
+                c.qprime = q;
+              } else {
+                c.eta = etaqp1;
+                c.qprime = q + 1;
+#pragma unroll
+                for (int j = 0; j <= Q; ++j)
+                  if (j == qmax)
+#pragma unroll
+                    for (int s = 0; s < RPL; ++s) zn[j][s] = acor[s];
+              }
+            }
+            if (c.eta < 1.5) {
+              c.eta = 1.0;
+              c.hprime = c.h;
+            } else {
+              c.eta = fmin(c.eta, c.etamax);
+              c.hprime = c.h * c.eta;
+              if (c.qprime < q) c.nscon = 0;
+            }
+          }
+          c.etamax = 10.0;
+          c.nstloc += 1;
+          const double tn = c.tn, h = c.h;
+          if ((tn - tout) * h >= 0.0) {
+            const double sfac = (tout - tn) / h;
+            double out[RPL];
+#pragma unroll
+            for (int s = 0; s < RPL; ++s) out[s] = 0.0;
+#pragma unroll
+            for (int j = Q; j >= 0; --j) {
+              if (j == q) {
+#pragma unroll
+                for (int s = 0; s < RPL; ++s) out[s] = zn[j][s];
+              } else if (j < q) {
+#pragma unroll
+                for (int s = 0; s < RPL; ++s) out[s] = zn[j][s] + sfac * out[s];
+              }
+            }
+            finish(ST_OK, out);
+          } else {
+            begin_step();
+          }
+        } else {
+          // error test failure
+          c.nef += 1;
+          c.netf += 1;
+          c.nflag = NF_ERR;
+          restore();
+          if (c.nef == 7) {
+            finish(ST_ERR_FAILURE, zn[0]);
+          } else {
+            c.etamax = 1.0;
+            if (c.nef <= 3) {
+              double eta = 1.0 / (pow(6.0 * dsm, 1.0 / c.L) + 1e-6);
+              eta = fmax(0.1, eta);
+              if (c.nef >= 2) eta = fmin(eta, 0.2);
+              c.eta = eta;
+              rescale();
+              attempt();
+            } else if (c.q > 1) {
+              c.eta = 0.1;
+              decrease_bdf();
+              c.L = c.q;
+              c.q -= 1;
+              c.qwait = c.L;
+              rescale();
+              attempt();
+            } else {
+              c.eta = 0.1;
+              c.h *= 0.1;
+              c.hscale = c.h;
+              c.qwait = 10;
+              c.nscon = 0;
+#pragma unroll
+              for (int s = 0; s < RPL; ++s) yv[s] = zn[0][s];
+              c.t_eval = c.tn;
+              c.phase = PH_RESTART;
+            }
+          }
+        }
+      }
+    }
+  }
+
+  if (lane == 0) {
+    atomicAdd(&P.stats[0], (unsigned long long)c.s_cells);
+    atomicAdd(&P.stats[1], (unsigned long long)c.s_nst);
+    atomicAdd(&P.stats[2], (unsigned long long)c.s_nfe);
+    atomicAdd(&P.stats[3], (unsigned long long)c.s_nje);
+    atomicAdd(&P.stats[4], (unsigned long long)c.s_nset);
+    atomicAdd(&P.stats[5], (unsigned long long)c.s_nni);
+    atomicAdd(&P.stats[6], (unsigned long long)c.s_netf);
+    atomicAdd(&P.stats[7], (unsigned long long)c.s_ncfn);
+    if (c.s_fail) atomicAdd(P.nfail, c.s_fail);
+  }
+}
+
+__global__ void zero_counters(unsigned int* queue, unsigned long long* stats, unsigned int* nfail) {
+  int t = threadIdx.x;
+  if (t < 8) stats[t] = 0ull;
+  if (t == 0) {
+    *queue = 0u;
+    *nfail = 0u;
+  }
+}
+
+}  // namespace prk
+#endif

@@ -1,0 +1,85 @@
+"""Mechanism data (species, weights, NASA-7 thermo, reactions) for the host side.
+
+The same spec modules feed the C header generator (``gen.py``) used by the HIP kernels; this
+module gives Python code (workload generators, tests) the thermodynamic tables in numpy form.
+"""
+from __future__ import annotations
+
+import importlib
+
+import numpy as np
+
+RU = 8.31446261815324e7   # erg / (mol K)
+PATM = 1013250.0          # dyn / cm^2
+ATOMIC_WEIGHTS = {"H": 1.008, "C": 12.011, "O": 15.999, "N": 14.007, "Ar": 39.95}
+
+_MODULES = {"lidryer": "lidryer", "drm19": "drm19"}
+_CACHE = {}
+
+
+class Mechanism:
+    def __init__(self, mod):
+        self.name = mod.NAME
+        self.species = list(mod.SPECIES)
+        self.nsp = len(self.species)
+        self.index = {s: i for i, s in enumerate(self.species)}
+        self.composition = mod.COMPOSITION
+        self.W = np.array([sum(ATOMIC_WEIGHTS[e] * c for e, c in mod.COMPOSITION[s].items())
+                           for s in self.species])
+        self.tmid = np.array([mod.THERMO[s][1] for s in self.species])
+        self.hi = np.array([mod.THERMO[s][3] for s in self.species])
+        self.lo = np.array([mod.THERMO[s][4] for s in self.species])
+        from .gen import parse_reactions
+        self.reactions = parse_reactions(mod.REACTIONS)
+        self.nr = len(self.reactions)
+        elems = [e for e in ATOMIC_WEIGHTS if any(mod.COMPOSITION[s].get(e, 0) for s in self.species)]
+        self.elements = elems
+        self.elem_matrix = np.array([[mod.COMPOSITION[s].get(e, 0) for e in elems]
+                                     for s in self.species], dtype=np.float64)
+
+    def _coef(self, T):
+        T = np.asarray(T, dtype=np.float64)
+        low = T[..., None] < self.tmid
+        return np.where(low[..., None], self.lo, self.hi), T[..., None]
+
+    def cp_R(self, T):
+        a, T = self._coef(T)
+        return a[..., 0] + T * (a[..., 1] + T * (a[..., 2] + T * (a[..., 3] + T * a[..., 4])))
+
+    def h_RT(self, T):
+        a, T = self._coef(T)
+        return (a[..., 0] + T * (a[..., 1] / 2 + T * (a[..., 2] / 3 + T * (a[..., 3] / 4 + T * a[..., 4] / 5)))
+                + a[..., 5] / T)
+
+    def s_R(self, T):
+        a, T = self._coef(T)
+        return (a[..., 0] * np.log(T) + T * (a[..., 1] + T * (a[..., 2] / 2 + T * (a[..., 3] / 3 + T * a[..., 4] / 4)))
+                + a[..., 6])
+
+    def h_mass(self, T, Y):
+        """Mixture enthalpy (erg/g) for T[...], Y[..., nsp]."""
+        T = np.asarray(T, dtype=np.float64)
+        return np.sum(np.asarray(Y) * self.h_RT(T) * RU * T[..., None] / self.W, axis=-1)
+
+    def cp_mass(self, T, Y):
+        return np.sum(np.asarray(Y) * self.cp_R(T) * RU / self.W, axis=-1)
+
+    def mean_W(self, Y):
+        return 1.0 / np.sum(np.asarray(Y) / self.W, axis=-1)
+
+    def density(self, P, T, Y):
+        return P * self.mean_W(Y) / (RU * np.asarray(T))
+
+    def X_to_Y(self, X):
+        X = np.asarray(X, dtype=np.float64)
+        m = X * self.W
+        return m / m.sum(axis=-1, keepdims=True)
+
+
+def load(name: str) -> Mechanism:
+    key = name.lower()
+    if key not in _MODULES:
+        raise KeyError(f"unknown mechanism {name}")
+    if key not in _CACHE:
+        _CACHE[key] = Mechanism(importlib.import_module(f"pelelmex_amd.mechanisms.{_MODULES[key]}"))
+    return _CACHE[key]

@@ -1,0 +1,180 @@
+"""Python mirror of PelePhysics' ReactorBase for the MI355X reactor (ctypes over the C-ABI).
+
+Reference interface (call sites in PeleLMeX):
+  ReactorBase::create(name)           Source/PeleLMeX_Setup.cpp:152-153
+  init(reactor_type, ncells)          Source/PeleLMeX_Setup.cpp:154
+  set_typ_vals_ode(typ)               Source/PeleLMeX_Utils.cpp:1472
+  react(box, rY, rYsrc, T, rE, rEsrc, FC, mask, dt_react, time, stream)
+                                      Source/PeleLMeX_Reactions.cpp:101-108
+  close()                             Source/PeleLMeX.cpp:26
+
+Device arrays are torch tensors on the GPU (plumbing only: the integration runs in
+libpelelmex_amd.so). Box arrays use the AMReX Array4 order, i.e. a C-contiguous tensor of
+shape [ncomp, nz, ny, nx] (or [nz, ny, nx] for one component). The library must be built;
+nothing here falls back to another implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpelelmex_amd.so")
+_LIB = None
+
+EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_get_stats", "pr_num_species",
+           "pr_destroy", "pr_last_error"]
+
+
+class Array4(C.Structure):
+    _fields_ = [("p", C.c_void_p), ("jstride", C.c_int64), ("kstride", C.c_int64),
+                ("nstride", C.c_int64), ("begin", C.c_int * 3), ("ncomp", C.c_int)]
+
+
+class IArray4(C.Structure):
+    _fields_ = [("p", C.c_void_p), ("jstride", C.c_int64), ("kstride", C.c_int64),
+                ("begin", C.c_int * 3)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("cells", C.c_int64), ("failed", C.c_int64), ("nst", C.c_int64), ("nfe", C.c_int64),
+                ("nje", C.c_int64), ("nsetups", C.c_int64), ("nni", C.c_int64), ("netf", C.c_int64),
+                ("ncfn", C.c_int64), ("kernel_ms", C.c_double), ("lanes_per_cell", C.c_int),
+                ("blocks", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def lib():
+    """Load libpelelmex_amd.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m pelelmex_amd.build` "
+                               "(or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.pr_create.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_char_p, C.POINTER(C.c_void_p)]
+        L.pr_create.restype = C.c_int
+        L.pr_set_typ_vals.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int]
+        L.pr_react_box.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(Array4),
+                                   C.POINTER(Array4), C.POINTER(Array4), C.POINTER(Array4), C.POINTER(Array4),
+                                   C.POINTER(Array4), C.POINTER(IArray4), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
+        L.pr_react_box.restype = C.c_int
+        L.pr_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+        L.pr_num_species.argtypes = [C.c_void_p]
+        L.pr_destroy.argtypes = [C.c_void_p]
+        L.pr_destroy.restype = None
+        L.pr_last_error.restype = C.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def _keys_text(keys):
+    if keys is None:
+        return None
+    if isinstance(keys, str):
+        return keys
+    return "\n".join(f"{k} = {v}" for k, v in keys.items())
+
+
+def _array4(t, ncomp):
+    if not t.is_contiguous():
+        raise ValueError("box arrays must be contiguous [ncomp, nz, ny, nx] / [nz, ny, nx]")
+    shp = t.shape[-3:]
+    nz, ny, nx = shp
+    return Array4(t.data_ptr(), nx, nx * ny, nx * ny * nz, (C.c_int * 3)(0, 0, 0), ncomp)
+
+
+class ReactorError(RuntimeError):
+    pass
+
+
+class Reactor:
+    """ReactorBase mirror: create(name) -> init(reactor_type, ncells) -> react(...) -> close()."""
+
+    def __init__(self, name, mechanism, keys=None):
+        self.name = name
+        self.mechanism = mechanism
+        self.keys = keys
+        self._h = None
+
+    @staticmethod
+    def create(name="ReactorCvode", mechanism="drm19", keys=None):
+        return Reactor(name, mechanism, keys)
+
+    def init(self, reactor_type=2, ncells=1):
+        h = C.c_void_p()
+        txt = _keys_text(self.keys)
+        rc = lib().pr_create(self.name.encode(), self.mechanism.encode(), int(reactor_type), int(ncells),
+                             txt.encode() if txt else None, C.byref(h))
+        if rc != 0:
+            raise ReactorError(f"pr_create failed ({rc}): {lib().pr_last_error().decode()}")
+        self._h = h
+        self.nsp = lib().pr_num_species(h)
+        return 0
+
+    def set_typ_vals_ode(self, typ):
+        import numpy as np
+        a = np.ascontiguousarray(typ, dtype=np.float64)
+        rc = lib().pr_set_typ_vals(self._h, a.ctypes.data_as(C.POINTER(C.c_double)), a.size)
+        if rc != 0:
+            raise ReactorError(lib().pr_last_error().decode())
+
+    def react(self, rY, rYsrc, T, rH, rHsrc, FC, mask, dt_react, time=0.0, stream=None, status=None):
+        """Box react on device tensors; returns (n_failed, dt_react, time)."""
+        import torch
+        if self._h is None:
+            raise ReactorError("react() before init()")
+        nz, ny, nx = T.shape
+        lo = (C.c_int * 3)(0, 0, 0)
+        hi = (C.c_int * 3)(nx - 1, ny - 1, nz - 1)
+        a = [_array4(rY, self.nsp), _array4(rYsrc, self.nsp), _array4(T, 1), _array4(rH, 1),
+             _array4(rHsrc, 1), _array4(FC, 1)]
+        m = None
+        if mask is not None:
+            m = IArray4(mask.data_ptr(), nx, nx * ny, (C.c_int * 3)(0, 0, 0))
+        dt = C.c_double(dt_react)
+        tm = C.c_double(time)
+        st = stream if stream is not None else torch.cuda.current_stream()
+        rc = lib().pr_react_box(self._h, lo, hi, *[C.byref(x) for x in a], C.byref(m) if m is not None else None,
+                                C.byref(dt), C.byref(tm), C.c_void_p(st.cuda_stream),
+                                C.c_void_p(status.data_ptr()) if status is not None else None)
+        if rc < 0:
+            raise ReactorError(f"pr_react_box failed ({rc}): {lib().pr_last_error().decode()}")
+        return rc, dt.value, tm.value
+
+    def stats(self):
+        s = Stats()
+        lib().pr_get_stats(self._h, C.byref(s))
+        return s.as_dict()
+
+    def close(self):
+        if self._h is not None:
+            lib().pr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def box_tensors(box, device="cuda"):
+    """Device tensors in Array4 order for a workload dict from pelelmex_amd.workloads."""
+    import torch
+    nx, ny, nz = box["dims"]
+    N = box["rY"].shape[0]
+
+    def t(a, nc):
+        x = torch.as_tensor(a, dtype=torch.float64)
+        x = x.reshape(nc, nz, ny, nx) if nc > 1 else x.reshape(nz, ny, nx)
+        return x.to(device).contiguous()
+
+    out = dict(rY=t(box["rY"], N), rYsrc=t(box["rYsrc"], N), T=t(box["T"], 1), rH=t(box["rH"], 1),
+               rHsrc=t(box["rHsrc"], 1))
+    out["FC"] = torch.zeros_like(out["T"])
+    out["mask"] = torch.as_tensor(box["mask"], dtype=torch.int32).reshape(nz, ny, nx).to(device).contiguous()
+    return out
