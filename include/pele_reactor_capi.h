@@ -17,6 +17,11 @@
  *   pr_get_stats     no reference counterpart: integration counters of the last react call
  *                    (FC carries RHS counts per cell, as PeleLMeX's functC does, PeleLMeX.H:209)
  *   pr_last_error    no reference counterpart: message for the last non-zero return code
+ *   pr_rhs_batch     the reactor right-hand side the integrator evaluates (PelePhysics' RHS kernel
+ *                    inside ReactorCvode, fed by RTY2WDOT as in Source/PeleLMeX_K.H:848-889)
+ *                    for a batch of states, one evaluation per cell
+ *   pr_jac_batch     the analytic Jacobian the integrator factors (PelePhysics' aJacobian path,
+ *                    PELE_COMPILE_AJACOBIAN in Exec/Make.PeleLMeX:100-104), one per cell
  */
 #ifndef PELE_REACTOR_CAPI_H
 #define PELE_REACTOR_CAPI_H
@@ -71,6 +76,14 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
                  const pr_array4* rYsrc, const pr_array4* T, const pr_array4* rH,
                  const pr_array4* rHsrc, const pr_array4* FC, const pr_iarray4* mask,
                  double* dt_react, double* time, void* stream, int* status_out);
+/* Batched evaluations on device arrays stored component-major with stride ncells:
+   y[i * ncells + c] (i < NUM_SPECIES + 1: rhoY then T), rYsrc[k * ncells + c], rhoh0[c],
+   rhohsrc[c]. f gets the right-hand side at time t (T recomputed from rho h(t) and Y, starting
+   from y_T); J[(i * n + j) * ncells + c] = d f_i / d y_j with T = y_T. Synchronous. */
+int pr_rhs_batch(pr_handle* h, int64_t ncells, const double* y, const double* rYsrc, const double* rhoh0,
+                 const double* rhohsrc, double t, double* f, void* stream);
+int pr_jac_batch(pr_handle* h, int64_t ncells, const double* y, const double* rYsrc, const double* rhohsrc,
+                 double* J, void* stream);
 int pr_get_stats(const pr_handle* h, pr_stats* out);
 int pr_num_species(const pr_handle* h);
 void pr_destroy(pr_handle* h);

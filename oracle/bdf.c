@@ -16,6 +16,7 @@
 #include <string.h>
 #include <float.h>
 #include "oracle.h"
+#include "prmath.h"
 
 #define QMAXC 5
 #define L_MAX (QMAXC + 1)
@@ -55,7 +56,7 @@ int orc_lu_factor(int n, double* A, int* piv) {
     for (int i = k + 1; i < n; ++i)
       if (fabs(A[i * n + k]) > fabs(A[p * n + k])) p = i;
     piv[k] = p;
-    if (A[p * n + k] == 0.0) return k + 1;
+    if (!(fabs(A[p * n + k]) > 0.0)) return k + 1;
     if (p != k)
       for (int j = 0; j < n; ++j) {
         double t = A[k * n + j];
@@ -66,12 +67,13 @@ int orc_lu_factor(int n, double* A, int* piv) {
     for (int i = k + 1; i < n; ++i) A[i * n + k] *= mult;
     for (int i = k + 1; i < n; ++i) {
       double lik = A[i * n + k];
-      for (int j = k + 1; j < n; ++j) A[i * n + j] -= lik * A[k * n + j];
+      for (int j = k + 1; j < n; ++j) A[i * n + j] = fma(-lik, A[k * n + j], A[i * n + j]);
     }
   }
   return 0;
 }
 
+/* Forward / back substitution; U_kk is applied as a multiplication by its reciprocal. */
 void orc_lu_solve(int n, const double* LU, const int* piv, double* b) {
   for (int k = 0; k < n; ++k) {
     int p = piv[k];
@@ -82,12 +84,11 @@ void orc_lu_solve(int n, const double* LU, const int* piv, double* b) {
     }
   }
   for (int k = 0; k < n - 1; ++k)
-    for (int i = k + 1; i < n; ++i) b[i] -= LU[i * n + k] * b[k];
-  for (int k = n - 1; k > 0; --k) {
-    b[k] /= LU[k * n + k];
-    for (int i = 0; i < k; ++i) b[i] -= LU[i * n + k] * b[k];
+    for (int i = k + 1; i < n; ++i) b[i] = fma(-LU[i * n + k], b[k], b[i]);
+  for (int k = n - 1; k >= 0; --k) {
+    b[k] = b[k] * (1.0 / LU[k * n + k]);
+    for (int i = 0; i < k; ++i) b[i] = fma(-LU[i * n + k], b[k], b[i]);
   }
-  b[0] /= LU[0];
 }
 
 typedef struct {
@@ -334,7 +335,7 @@ static int nls(bdf_t* b, int nflag) {
   }
 }
 
-static double etaq_of(double dsm, int L) { return 1.0 / (pow(BIAS2 * dsm, 1.0 / L) + ADDON); }
+static double etaq_of(double dsm, int L) { return 1.0 / (prm_pow(BIAS2 * dsm, 1.0 / L) + ADDON); }
 
 static int step(bdf_t* b) {
   double saved_t = b->tn;
@@ -420,14 +421,14 @@ static int step(bdf_t* b) {
       b->etaqm1 = 0.0;
       if (b->q > 1) {
         double ddn = wrms(b, b->zn[b->q]) * b->tq[1];
-        b->etaqm1 = 1.0 / (pow(BIAS1 * ddn, 1.0 / b->q) + ADDON);
+        b->etaqm1 = 1.0 / (prm_pow(BIAS1 * ddn, 1.0 / b->q) + ADDON);
       }
       b->etaqp1 = 0.0;
       if (b->q != b->qmax && b->saved_tq5 != 0.0) {
-        double cquot = (b->tq[5] / b->saved_tq5) * pow(b->h / b->tau[2], b->L);
+        double cquot = (b->tq[5] / b->saved_tq5) * prm_pow(b->h / b->tau[2], (double)b->L);
         for (int i = 0; i < b->n; ++i) b->tempv[i] = b->acor[i] - cquot * b->zn[b->qmax][i];
         double dup = wrms(b, b->tempv) * b->tq[3];
-        b->etaqp1 = 1.0 / (pow(BIAS3 * dup, 1.0 / (b->L + 1)) + ADDON);
+        b->etaqp1 = 1.0 / (prm_pow(BIAS3 * dup, 1.0 / (b->L + 1)) + ADDON);
       }
       double etam = fmax(b->etaqm1, fmax(b->etaq, b->etaqp1));
       if (etam < THRESH) {

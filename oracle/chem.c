@@ -1,8 +1,7 @@
 /* CPU oracle: thermodynamics, reaction rates, reactor right-hand side and its Jacobian, and
  * the box-level react driver.
  *
- * TEST INFRASTRUCTURE ONLY (see oracle.h). Plain loops over the mechanism tables, written
- * reaction-major (scatter), independently of the species-major gather the HIP kernels use.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
  *
  * Restates (bodies absent from the reference tree; semantics from the call sites and docs):
  *  - the reactor right-hand side PelePhysics evaluates for reactor_type = 2 (constant pressure,
@@ -16,15 +15,20 @@
  *    here the exact Jacobian of the right-hand side with T taken as a state variable.
  *  - react over a box with mask == -1 cells skipped and FC = number of RHS evaluations
  *    (Source/PeleLMeX_Reactions.cpp:101-108, Source/PeleLMeX_EB.cpp:157-161, PeleLMeX.H:209).
+ *
+ * Arithmetic is written as explicit IEEE-754 expression trees (no implicit contraction, fma
+ * where stated) with sums taken in component order and production rates / Jacobian rows
+ * gathered species by species over each species' reactions in ascending order. The HIP
+ * kernels evaluate the same trees, so GPU and oracle agree bit for bit; the oracle itself is
+ * pinned by tests against finite differences, conservation laws and scipy integrators.
  */
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #include "oracle.h"
+#include "prmath.h"
 #include "../pelelmex_amd/csrc/mech/lidryer.h"
 #include "../pelelmex_amd/csrc/mech/drm19.h"
-
-#define LN10 2.302585092994045684
 
 static const pr_mech_t* const MECHS[] = {&pr_lidryer, &pr_drm19};
 #define NMECH ((int)(sizeof(MECHS) / sizeof(MECHS[0])))
@@ -42,20 +46,86 @@ void orc_mech_weights(int id, double* W) {
   for (int k = 0; k < MECHS[id]->nsp; ++k) W[k] = MECHS[id]->W[k];
 }
 
-/* NASA-7: cp/R, h/RT, s/R and d(cp/R)/dT of one species */
-static void nasa(const double* c15, double T, double* cp, double* h, double* s, double* dcp) {
-  const double* a = (T < c15[0]) ? c15 + 8 : c15 + 1;
-  double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
-  if (cp) *cp = a[0] + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4;
-  if (h) *h = a[0] + a[1] * T / 2 + a[2] * T2 / 3 + a[3] * T3 / 4 + a[4] * T4 / 5 + a[5] / T;
-  if (s) *s = a[0] * log(T) + a[1] * T + a[2] * T2 / 2 + a[3] * T3 / 3 + a[4] * T4 / 4 + a[6];
-  if (dcp) *dcp = a[1] + 2 * a[2] * T + 3 * a[3] * T2 + 4 * a[4] * T3;
+/* Derived per-species tables: R/W, 1/W and the NASA-7 polynomials rewritten for Horner
+   evaluation of cp/R, h/RT (without the a6/T term), s/R (without a1 ln T and a7) and d(cp/R)/dT.
+   Range 0 = low (T < T_mid), 1 = high. */
+typedef struct {
+  double RW, invW, W, tmid;
+  double cp[2][5], h[2][6], s[2][6], dcp[2][4];
+} sp_t;
+
+static sp_t SP[NMECH][ORC_NMAX];
+static int SP_READY = 0;
+
+static void derive_species(void) {
+  for (int id = 0; id < NMECH; ++id) {
+    const pr_mech_t* m = MECHS[id];
+    for (int k = 0; k < m->nsp; ++k) {
+      sp_t* o = &SP[id][k];
+      o->W = m->W[k];
+      o->RW = PR_RU / m->W[k];
+      o->invW = 1.0 / m->W[k];
+      o->tmid = m->nasa[15 * k];
+      for (int r = 0; r < 2; ++r) {
+        const double* a = m->nasa + 15 * k + (r == 0 ? 8 : 1);
+        for (int c = 0; c < 5; ++c) o->cp[r][c] = a[c];
+        o->h[r][0] = a[0];
+        o->h[r][1] = a[1] / 2.0;
+        o->h[r][2] = a[2] / 3.0;
+        o->h[r][3] = a[3] / 4.0;
+        o->h[r][4] = a[4] / 5.0;
+        o->h[r][5] = a[5];
+        o->s[r][0] = a[0];
+        o->s[r][1] = a[1];
+        o->s[r][2] = a[2] / 2.0;
+        o->s[r][3] = a[3] / 3.0;
+        o->s[r][4] = a[4] / 4.0;
+        o->s[r][5] = a[6];
+        o->dcp[r][0] = a[1];
+        o->dcp[r][1] = 2.0 * a[2];
+        o->dcp[r][2] = 3.0 * a[3];
+        o->dcp[r][3] = 4.0 * a[4];
+      }
+    }
+  }
+  SP_READY = 1;
+}
+
+__attribute__((constructor)) static void orc_init(void) { derive_species(); }
+
+/* cp/R and h/RT at T */
+static void nasa_hcp(const sp_t* p, double T, double invT, double* cpR, double* hRT) {
+  int r = (T < p->tmid) ? 0 : 1;
+  const double* c = p->cp[r];
+  const double* h = p->h[r];
+  *cpR = fma(fma(fma(fma(c[4], T, c[3]), T, c[2]), T, c[1]), T, c[0]);
+  double hp = fma(fma(fma(fma(h[4], T, h[3]), T, h[2]), T, h[1]), T, h[0]);
+  *hRT = fma(h[5], invT, hp);
+}
+
+/* cp/R, h/RT, g/RT = h/RT - s/R and d(cp/R)/dT at T */
+static void nasa_full(const sp_t* p, double T, double invT, double lnT, double* cpR, double* hRT,
+                      double* gRT, double* dcpR) {
+  int r = (T < p->tmid) ? 0 : 1;
+  nasa_hcp(p, T, invT, cpR, hRT);
+  const double* s = p->s[r];
+  const double* d = p->dcp[r];
+  double sp = fma(fma(fma(s[4], T, s[3]), T, s[2]), T, s[1]) * T;
+  double sR = fma(s[0], lnT, sp + s[5]);
+  *gRT = *hRT - sR;
+  *dcpR = fma(fma(fma(d[3], T, d[2]), T, d[1]), T, d[0]);
 }
 
 void orc_thermo(int id, double T, double* cpR, double* hRT, double* sR) {
   const pr_mech_t* m = MECHS[id];
-  for (int k = 0; k < m->nsp; ++k)
-    nasa(m->nasa + 15 * k, T, cpR ? cpR + k : NULL, hRT ? hRT + k : NULL, sR ? sR + k : NULL, NULL);
+  double invT = 1.0 / T, lnT = prm_log(T);
+  for (int k = 0; k < m->nsp; ++k) {
+    double c, h, g, d;
+    nasa_full(&SP[id][k], T, invT, lnT, &c, &h, &g, &d);
+    if (cpR) cpR[k] = c;
+    if (hRT) hRT[k] = h;
+    if (sR) sR[k] = h - g;
+  }
 }
 
 /* Per-reaction quantities needed by both the rates and the Jacobian. */
@@ -66,136 +136,209 @@ typedef struct {
   double dqdT;       /* d q / d T at constant concentrations */
 } rx_info;
 
-static void reaction(const pr_mech_t* m, int r, const double* C, double Ctot, double T,
-                     const double* hRT, const double* gRT, rx_info* o) {
-  double lnT = log(T), invT = 1.0 / T;
+static void reaction(const pr_mech_t* m, int r, const double* C, double Ctot, double T, double lnT,
+                     double invT, double lnRTP, const double* gRT, const double* hRT, int jac,
+                     rx_info* o) {
   const double* a = m->arr + 3 * r;
-  double kinf = exp(a[0] + a[1] * lnT - a[2] * invT);
-  double dlnkinf = (a[1] + a[2] * invT) * invT;
-  int type = m->rtype[r];
+  const int type = m->rtype[r], nre = m->nreac[r], npr = m->nprod[r];
+  const int* rs = m->reac + 3 * r;
+  const int* ps = m->prod + 3 * r;
+  double kinf = prm_exp(fma(-a[2], invT, fma(a[1], lnT, a[0])));
+  double dlnkinf = fma(a[2], invT, a[1]) * invT;
   double M = 0.0;
   if (type != PR_RX_ELEM) {
     M = m->eff_dflt[r] * Ctot;
-    for (int e = m->eff_start[r]; e < m->eff_start[r + 1]; ++e) M += m->eff_val[e] * C[m->eff_sp[e]];
+    for (int e = m->eff_start[r]; e < m->eff_start[r + 1]; ++e) M = fma(m->eff_val[e], C[m->eff_sp[e]], M);
   }
-  double pr = 1.0, pp = 1.0;
-  for (int s = 0; s < m->nreac[r]; ++s) pr *= C[m->reac[3 * r + s]];
-  for (int s = 0; s < m->nprod[r]; ++s) pp *= C[m->prod[3 * r + s]];
+  double pr = C[rs[0]];
+  if (nre > 1) pr *= C[rs[1]];
+  if (nre > 2) pr *= C[rs[2]];
+  double pp = C[ps[0]];
+  if (npr > 1) pp *= C[ps[1]];
+  if (npr > 2) pp *= C[ps[2]];
   double invKc = 0.0, dH = 0.0;
-  int sumnu = m->nprod[r] - m->nreac[r];
+  const int sumnu = npr - nre;
   if (m->rev[r]) {
-    double dG = 0.0;
-    for (int s = 0; s < m->nprod[r]; ++s) {
-      dG += gRT[m->prod[3 * r + s]];
-      dH += hRT[m->prod[3 * r + s]];
+    double dG = gRT[ps[0]];
+    if (npr > 1) dG += gRT[ps[1]];
+    if (npr > 2) dG += gRT[ps[2]];
+    dG -= gRT[rs[0]];
+    if (nre > 1) dG -= gRT[rs[1]];
+    if (nre > 2) dG -= gRT[rs[2]];
+    invKc = prm_exp(fma((double)sumnu, lnRTP, dG));
+    if (jac) {
+      dH = hRT[ps[0]];
+      if (npr > 1) dH += hRT[ps[1]];
+      if (npr > 2) dH += hRT[ps[2]];
+      dH -= hRT[rs[0]];
+      if (nre > 1) dH -= hRT[rs[1]];
+      if (nre > 2) dH -= hRT[rs[2]];
     }
-    for (int s = 0; s < m->nreac[r]; ++s) {
-      dG -= gRT[m->reac[3 * r + s]];
-      dH -= hRT[m->reac[3 * r + s]];
-    }
-    invKc = exp(dG + sumnu * log(PR_RU * T / PR_PATM));
   }
-  double dlnKc_T = (dH - sumnu) * invT; /* d ln K_c / dT */
+  const double dlnKc = (dH - (double)sumnu) * invT;
   if (type == PR_RX_ELEM || type == PR_RX_THIRDBODY) {
-    double kf0 = kinf, kr0 = kinf * invKc;
-    double fwd0 = kf0 * pr, rev0 = kr0 * pp;
-    double dq0dT = fwd0 * dlnkinf - rev0 * (dlnkinf - dlnKc_T);
+    double kr0 = kinf * invKc;
+    double fwd0 = kinf * pr, rev0 = kr0 * pp;
+    double dq0 = 0.0;
+    if (jac) dq0 = fwd0 * dlnkinf - rev0 * (dlnkinf - dlnKc);
     if (type == PR_RX_ELEM) {
-      o->kf = kf0; o->kr = kr0; o->fwd = fwd0; o->rev = rev0; o->dqdM = 0.0; o->dqdT = dq0dT;
+      o->kf = kinf; o->kr = kr0; o->fwd = fwd0; o->rev = rev0; o->dqdM = 0.0; o->dqdT = dq0;
     } else {
-      o->kf = M * kf0; o->kr = M * kr0; o->fwd = M * fwd0; o->rev = M * rev0;
-      o->dqdM = fwd0 - rev0; o->dqdT = M * dq0dT;
+      o->kf = M * kinf; o->kr = M * kr0; o->fwd = M * fwd0; o->rev = M * rev0;
+      o->dqdM = fwd0 - rev0; o->dqdT = M * dq0;
     }
     return;
   }
   /* falloff */
   const double* lo = m->low + 3 * r;
-  double k0 = exp(lo[0] + lo[1] * lnT - lo[2] * invT);
-  double dlnk0 = (lo[1] + lo[2] * invT) * invT;
-  double Pr = k0 * M / kinf;
+  double k0 = prm_exp(fma(-lo[2], invT, fma(lo[1], lnT, lo[0])));
+  double Pr = (k0 * M) / kinf;
   if (Pr < 1e-300) Pr = 1e-300;
-  double F = 1.0, dlogF_dx = 0.0, dlogF_dT = 0.0;
-  double dlnPr = dlnk0 - dlnkinf;
+  double F = 1.0, dlogF_dx = 0.0, dlogF_dT = 0.0, dlnPr = 0.0;
+  if (jac) dlnPr = fma(lo[2], invT, lo[1]) * invT - dlnkinf;
   if (type == PR_RX_TROE) {
     const double* tr = m->troe + 4 * r;
-    double e3 = exp(-T / tr[1]), e1 = exp(-T / tr[2]);
+    double e3 = prm_exp(-T / tr[1]), e1 = prm_exp(-T / tr[2]);
     double Fc = (1.0 - tr[0]) * e3 + tr[0] * e1;
-    double dFc = -(1.0 - tr[0]) / tr[1] * e3 - tr[0] / tr[2] * e1;
+    double dFc = 0.0;
+    if (jac) dFc = -((1.0 - tr[0]) / tr[1]) * e3 - (tr[0] / tr[2]) * e1;
     if (m->ntroe[r] == 4) {
-      double e2 = exp(-tr[3] * invT);
+      double e2 = prm_exp(-tr[3] * invT);
       Fc += e2;
-      dFc += tr[3] * invT * invT * e2;
+      if (jac) dFc += ((tr[3] * invT) * invT) * e2;
     }
     if (Fc < 1e-300) Fc = 1e-300;
-    double logFc = log10(Fc);
+    double logFc = prm_log10(Fc);
     double c = -0.4 - 0.67 * logFc, nn = 0.75 - 1.27 * logFc;
-    double x = log10(Pr);
+    double x = prm_log10(Pr);
     double xc = x + c, D = nn - 0.14 * xc, f1 = xc / D;
     double den = 1.0 + f1 * f1;
-    double logF = logFc / den;
-    F = exp(logF * LN10);
-    dlogF_dx = -logFc * 2.0 * f1 * (nn / (D * D)) / (den * den);
-    double dlogFc = dFc / (Fc * LN10);
-    double dx = dlnPr / LN10, dc = -0.67 * dlogFc, dnn = -1.27 * dlogFc;
-    double dxc = dx + dc, dD = dnn - 0.14 * dxc;
-    double df1 = (dxc * D - xc * dD) / (D * D);
-    dlogF_dT = (dlogFc * den - logFc * 2.0 * f1 * df1) / (den * den);
+    F = prm_exp((logFc / den) * PRM_LN10);
+    if (jac) {
+      dlogF_dx = -logFc * 2.0 * f1 * (nn / (D * D)) / (den * den);
+      double dlogFc = dFc / (Fc * PRM_LN10);
+      double dx = dlnPr / PRM_LN10, dc = -0.67 * dlogFc, dnn = -1.27 * dlogFc;
+      double dxc = dx + dc, dD = dnn - 0.14 * dxc;
+      double df1 = (dxc * D - xc * dD) / (D * D);
+      dlogF_dT = (dlogFc * den - logFc * 2.0 * f1 * df1) / (den * den);
+    }
   }
-  double kf = kinf * Pr / (1.0 + Pr) * F;
+  double kf = ((kinf * Pr) / (1.0 + Pr)) * F;
   double kr = kf * invKc;
-  double dlnkf = dlnkinf + dlnPr / (1.0 + Pr) + LN10 * dlogF_dT;
   o->kf = kf;
   o->kr = kr;
   o->fwd = kf * pr;
   o->rev = kr * pp;
-  o->dqdM = k0 * F * (1.0 / ((1.0 + Pr) * (1.0 + Pr)) + dlogF_dx / (1.0 + Pr)) * (pr - pp * invKc);
-  o->dqdT = o->fwd * dlnkf - o->rev * (dlnkf - dlnKc_T);
+  o->dqdM = 0.0;
+  o->dqdT = 0.0;
+  if (jac) {
+    double dlnkf = dlnkinf + dlnPr / (1.0 + Pr) + PRM_LN10 * dlogF_dT;
+    double onep = 1.0 + Pr;
+    o->dqdM = ((k0 * F) * (1.0 / (onep * onep) + dlogF_dx / onep)) * (pr - pp * invKc);
+    o->dqdT = o->fwd * dlnkf - o->rev * (dlnkf - dlnKc);
+  }
 }
 
-static void gibbs(const pr_mech_t* m, double T, double* cpR, double* hRT, double* gRT, double* dcpR) {
+/* Thermo of every species at T into per-species arrays; concentrations from rhoY. */
+typedef struct {
+  double T, lnT, invT, lnRTP;
+  double C[ORC_NMAX], gRT[ORC_NMAX], hRT[ORC_NMAX], hk[ORC_NMAX], cpk[ORC_NMAX], dcpk[ORC_NMAX];
+  double Ctot;
+} state_t;
+
+static void state_at(int id, const double* y, double T, state_t* s) {
+  const pr_mech_t* m = MECHS[id];
+  s->T = T;
+  s->lnT = prm_log(T);
+  s->invT = 1.0 / T;
+  s->lnRTP = prm_log((PR_RU * T) / PR_PATM);
+  s->Ctot = 0.0;
   for (int k = 0; k < m->nsp; ++k) {
-    double s;
-    nasa(m->nasa + 15 * k, T, cpR ? cpR + k : NULL, hRT + k, &s, dcpR ? dcpR + k : NULL);
-    gRT[k] = hRT[k] - s;
+    const sp_t* p = &SP[id][k];
+    double cpR, hRT, gRT, dcpR;
+    nasa_full(p, T, s->invT, s->lnT, &cpR, &hRT, &gRT, &dcpR);
+    s->C[k] = y[k] * p->invW;
+    s->gRT[k] = gRT;
+    s->hRT[k] = hRT;
+    s->hk[k] = (hRT * p->RW) * T;
+    s->cpk[k] = cpR * p->RW;
+    s->dcpk[k] = dcpR * p->RW;
+  }
+  for (int k = 0; k < m->nsp; ++k) s->Ctot += s->C[k];
+}
+
+/* production rates by the species-wise gather over each species' reactions */
+static void gather_wdot(const pr_mech_t* m, const double* q, double* wdot) {
+  for (int k = 0; k < m->nsp; ++k) {
+    double w = 0.0;
+    for (int e = m->inc_start[k]; e < m->inc_start[k + 1]; ++e) w += (double)m->inc_nu[e] * q[m->inc_r[e]];
+    wdot[k] = w;
   }
 }
 
 void orc_rates(int id, const double* C, double T, double* q, double* wdot) {
   const pr_mech_t* m = MECHS[id];
-  double hRT[ORC_NMAX], gRT[ORC_NMAX], Ctot = 0.0;
-  gibbs(m, T, NULL, hRT, gRT, NULL);
-  for (int k = 0; k < m->nsp; ++k) Ctot += C[k];
-  if (wdot) memset(wdot, 0, sizeof(double) * m->nsp);
+  double y[ORC_NMAX] = {0}, qq[1024];
+  for (int k = 0; k < m->nsp; ++k) y[k] = C[k] * m->W[k];
+  state_t s;
+  state_at(id, y, T, &s);
+  /* use the given concentrations exactly */
+  s.Ctot = 0.0;
+  for (int k = 0; k < m->nsp; ++k) {
+    s.C[k] = C[k];
+    s.Ctot += C[k];
+  }
   for (int r = 0; r < m->nr; ++r) {
     rx_info o;
-    reaction(m, r, C, Ctot, T, hRT, gRT, &o);
-    double qr = o.fwd - o.rev;
-    if (q) q[r] = qr;
-    if (wdot) {
-      for (int s = 0; s < m->nreac[r]; ++s) wdot[m->reac[3 * r + s]] -= qr;
-      for (int s = 0; s < m->nprod[r]; ++s) wdot[m->prod[3 * r + s]] += qr;
-    }
+    reaction(m, r, s.C, s.Ctot, T, s.lnT, s.invT, s.lnRTP, s.gRT, s.hRT, 0, &o);
+    qq[r] = o.fwd - o.rev;
+    if (q) q[r] = qq[r];
+  }
+  if (wdot) gather_wdot(m, qq, wdot);
+}
+
+void orc_rates_fr(int id, const double* C, double T, double* fwd, double* rev) {
+  const pr_mech_t* m = MECHS[id];
+  double y[ORC_NMAX] = {0};
+  for (int k = 0; k < m->nsp; ++k) y[k] = C[k] * m->W[k];
+  state_t s;
+  state_at(id, y, T, &s);
+  s.Ctot = 0.0;
+  for (int k = 0; k < m->nsp; ++k) {
+    s.C[k] = C[k];
+    s.Ctot += C[k];
+  }
+  for (int r = 0; r < m->nr; ++r) {
+    rx_info o;
+    reaction(m, r, s.C, s.Ctot, T, s.lnT, s.invT, s.lnRTP, s.gRT, s.hRT, 0, &o);
+    fwd[r] = o.fwd;
+    rev[r] = o.rev;
   }
 }
+
+double orc_exp(double x) { return prm_exp(x); }
+double orc_log(double x) { return prm_log(x); }
 
 #define HY2T_MAXIT 20
 #define HY2T_RTOL 1e-10
 
-int orc_hy2t(int id, double h, const double* Y, double* T) {
+/* Newton on T for h(T, Y) = h; Y given as rhoY and rho (Y_k = rhoY_k / rho). */
+static int hy2t_rhoY(int id, double h, const double* rY, double rho, double* T) {
   const pr_mech_t* m = MECHS[id];
   double t = *T;
   int it;
   for (it = 0; it < HY2T_MAXIT; ++it) {
-    double hm = 0.0, cpm = 0.0;
+    double invT = 1.0 / t, hm = 0.0, cpm = 0.0;
     for (int k = 0; k < m->nsp; ++k) {
-      double cp, hr;
-      nasa(m->nasa + 15 * k, t, &cp, &hr, NULL, NULL);
-      double RW = PR_RU / m->W[k];
-      hm += Y[k] * hr * RW * t;
-      cpm += Y[k] * cp * RW;
+      const sp_t* p = &SP[id][k];
+      double cpR, hRT;
+      nasa_hcp(p, t, invT, &cpR, &hRT);
+      double Y = rY[k] / rho;
+      hm += ((Y * hRT) * p->RW) * t;
+      cpm += (Y * cpR) * p->RW;
     }
     double dT = (h - hm) / cpm;
-    t += dT;
+    t = t + dT;
     if (fabs(dT) <= HY2T_RTOL * t) {
       ++it;
       break;
@@ -205,41 +348,44 @@ int orc_hy2t(int id, double h, const double* Y, double* T) {
   return it;
 }
 
+int orc_hy2t(int id, double h, const double* Y, double* T) { return hy2t_rhoY(id, h, Y, 1.0, T); }
+
 /* Shared tail of both right-hand sides, evaluated at temperature T. */
-static void rhs_at_T(const pr_mech_t* m, const orc_cell* c, const double* y, double T, double* f) {
-  int N = m->nsp;
-  double C[ORC_NMAX], wdot[ORC_NMAX], hRT[ORC_NMAX], cpR[ORC_NMAX];
-  for (int k = 0; k < N; ++k) C[k] = y[k] / m->W[k];
-  orc_rates(orc_mech_find(m->name), C, T, NULL, wdot);
-  orc_thermo(orc_mech_find(m->name), T, cpR, hRT, NULL);
-  double num = c->rhohsrc, rcp = 0.0;
-  for (int k = 0; k < N; ++k) {
-    double RW = PR_RU / m->W[k];
-    f[k] = wdot[k] * m->W[k] + c->rYsrc[k];
-    num -= hRT[k] * RW * T * f[k];
-    rcp += y[k] * cpR[k] * RW;
+static void rhs_at_T(int id, const orc_cell* c, const double* y, double T, double* f) {
+  const pr_mech_t* m = MECHS[id];
+  const int N = m->nsp;
+  state_t s;
+  double q[1024], wdot[ORC_NMAX];
+  state_at(id, y, T, &s);
+  for (int r = 0; r < m->nr; ++r) {
+    rx_info o;
+    reaction(m, r, s.C, s.Ctot, T, s.lnT, s.invT, s.lnRTP, s.gRT, s.hRT, 0, &o);
+    q[r] = o.fwd - o.rev;
   }
-  f[N] = num / rcp;
+  gather_wdot(m, q, wdot);
+  for (int k = 0; k < N; ++k) f[k] = wdot[k] * m->W[k] + c->rYsrc[k];
+  double pn = 0.0, rcp = 0.0;
+  for (int k = 0; k < N; ++k) pn += s.hk[k] * f[k];
+  for (int k = 0; k < N; ++k) rcp += y[k] * s.cpk[k];
+  f[N] = (c->rhohsrc - pn) / rcp;
 }
 
 int orc_reactor_rhs(double t, const double* y, double* f, void* cell) {
   const orc_cell* c = (const orc_cell*)cell;
-  const pr_mech_t* m = MECHS[c->mech];
-  int N = m->nsp;
-  double rho = 0.0, Y[ORC_NMAX];
+  const int N = MECHS[c->mech]->nsp;
+  double rho = 0.0;
   for (int k = 0; k < N; ++k) rho += y[k];
-  for (int k = 0; k < N; ++k) Y[k] = y[k] / rho;
   double h = (c->rhoh0 + t * c->rhohsrc) / rho;
   double T = y[N];
-  orc_hy2t(c->mech, h, Y, &T);
-  rhs_at_T(m, c, y, T, f);
+  hy2t_rhoY(c->mech, h, y, rho, &T);
+  rhs_at_T(c->mech, c, y, T, f);
   return 0;
 }
 
 int orc_reactor_rhs_frozen(double t, const double* y, double* f, void* cell) {
   (void)t;
   const orc_cell* c = (const orc_cell*)cell;
-  rhs_at_T(MECHS[c->mech], c, y, y[MECHS[c->mech]->nsp], f);
+  rhs_at_T(c->mech, c, y, y[MECHS[c->mech]->nsp], f);
   return 0;
 }
 
@@ -247,80 +393,62 @@ int orc_reactor_jac(double t, const double* y, const double* fy, double* J, void
   (void)t;
   (void)fy;
   const orc_cell* c = (const orc_cell*)cell;
-  const pr_mech_t* m = MECHS[c->mech];
-  int N = m->nsp, n = N + 1;
-  double T = y[N];
-  double C[ORC_NMAX], hRT[ORC_NMAX], gRT[ORC_NMAX], cpR[ORC_NMAX], dcpR[ORC_NMAX];
-  double JC[ORC_NMAX * ORC_NMAX]; /* d wdot_k / d C_m (m < N), d wdot_k / dT (m = N) */
-  double dq[ORC_NMAX], wdot[ORC_NMAX];
-  gibbs(m, T, cpR, hRT, gRT, dcpR);
-  double Ctot = 0.0;
+  const int id = c->mech;
+  const pr_mech_t* m = MECHS[id];
+  const int N = m->nsp, n = N + 1;
+  const double T = y[N];
+  state_t s;
+  state_at(id, y, T, &s);
+  static __thread rx_info rx[1024];
+  for (int r = 0; r < m->nr; ++r)
+    reaction(m, r, s.C, s.Ctot, T, s.lnT, s.invT, s.lnRTP, s.gRT, s.hRT, 1, &rx[r]);
+  double fk[ORC_NMAX];
+  /* species rows: gather over each species' reactions */
   for (int k = 0; k < N; ++k) {
-    C[k] = y[k] / m->W[k];
-    Ctot += C[k];
-  }
-  memset(JC, 0, sizeof(double) * N * n);
-  memset(wdot, 0, sizeof(double) * N);
-  for (int r = 0; r < m->nr; ++r) {
-    rx_info o;
-    reaction(m, r, C, Ctot, T, hRT, gRT, &o);
-    /* dense dq/dC for this reaction */
-    for (int k = 0; k < N; ++k) dq[k] = 0.0;
-    if (m->rtype[r] != PR_RX_ELEM) {
-      for (int k = 0; k < N; ++k) dq[k] = o.dqdM * m->eff_dflt[r];
-      for (int e = m->eff_start[r]; e < m->eff_start[r + 1]; ++e) dq[m->eff_sp[e]] += o.dqdM * m->eff_val[e];
+    double* row = J + k * n;
+    for (int j = 0; j < n; ++j) row[j] = 0.0;
+    double Sd = 0.0, JT = 0.0, w = 0.0;
+    for (int e = m->inc_start[k]; e < m->inc_start[k + 1]; ++e) {
+      const int r = m->inc_r[e];
+      const double nu = (double)m->inc_nu[e];
+      const rx_info* o = &rx[r];
+      const int nre = m->nreac[r], npr = m->nprod[r];
+      const int* rs = m->reac + 3 * r;
+      const int* ps = m->prod + 3 * r;
+      w += nu * (o->fwd - o->rev);
+      double c0 = s.C[rs[0]], c1 = nre > 1 ? s.C[rs[1]] : 1.0, c2 = nre > 2 ? s.C[rs[2]] : 1.0;
+      double kf = nu * o->kf;
+      row[rs[0]] += (kf * c1) * c2;
+      if (nre > 1) row[rs[1]] += (kf * c0) * c2;
+      if (nre > 2) row[rs[2]] += (kf * c0) * c1;
+      double d0 = s.C[ps[0]], d1 = npr > 1 ? s.C[ps[1]] : 1.0, d2 = npr > 2 ? s.C[ps[2]] : 1.0;
+      double kr = nu * o->kr;
+      row[ps[0]] -= (kr * d1) * d2;
+      if (npr > 1) row[ps[1]] -= (kr * d0) * d2;
+      if (npr > 2) row[ps[2]] -= (kr * d0) * d1;
+      if (m->rtype[r] != PR_RX_ELEM) {
+        double dm = nu * o->dqdM;
+        Sd += dm * m->eff_dflt[r];
+        for (int q = m->eff_start[r]; q < m->eff_start[r + 1]; ++q) row[m->eff_sp[q]] += dm * m->eff_val[q];
+      }
+      JT += nu * o->dqdT;
     }
-    int nre = m->nreac[r], npr = m->nprod[r];
-    for (int s = 0; s < nre; ++s) {
-      double p = o.kf;
-      for (int s2 = 0; s2 < nre; ++s2)
-        if (s2 != s) p *= C[m->reac[3 * r + s2]];
-      dq[m->reac[3 * r + s]] += p;
-    }
-    for (int s = 0; s < npr; ++s) {
-      double p = o.kr;
-      for (int s2 = 0; s2 < npr; ++s2)
-        if (s2 != s) p *= C[m->prod[3 * r + s2]];
-      dq[m->prod[3 * r + s]] -= p;
-    }
-    dq[N] = o.dqdT;
-    double qr = o.fwd - o.rev;
-    /* scatter with the stoichiometry of each molecule slot */
-    for (int s = 0; s < nre; ++s) {
-      int k = m->reac[3 * r + s];
-      wdot[k] -= qr;
-      for (int j = 0; j < n; ++j) JC[k * n + j] -= dq[j];
-    }
-    for (int s = 0; s < npr; ++s) {
-      int k = m->prod[3 * r + s];
-      wdot[k] += qr;
-      for (int j = 0; j < n; ++j) JC[k * n + j] += dq[j];
-    }
+    const double Wk = m->W[k];
+    for (int j = 0; j < N; ++j) row[j] = ((row[j] + Sd) * Wk) * SP[id][j].invW;
+    row[N] = JT * Wk;
+    fk[k] = w * Wk + c->rYsrc[k];
   }
-  /* map to (rhoY, T) variables and append the temperature row */
-  double fk[ORC_NMAX], hk[ORC_NMAX], cpk[ORC_NMAX], rcp = 0.0, num = c->rhohsrc, sdcp = 0.0;
-  for (int k = 0; k < N; ++k) {
-    double RW = PR_RU / m->W[k];
-    fk[k] = wdot[k] * m->W[k] + c->rYsrc[k];
-    hk[k] = hRT[k] * RW * T;
-    cpk[k] = cpR[k] * RW;
-    rcp += y[k] * cpk[k];
-    num -= hk[k] * fk[k];
-    sdcp += y[k] * dcpR[k] * RW;
+  double pn = 0.0, rcp = 0.0, sdcp = 0.0, scf = 0.0;
+  for (int k = 0; k < N; ++k) pn += s.hk[k] * fk[k];
+  for (int k = 0; k < N; ++k) rcp += y[k] * s.cpk[k];
+  for (int k = 0; k < N; ++k) sdcp += y[k] * s.dcpk[k];
+  for (int k = 0; k < N; ++k) scf += s.cpk[k] * fk[k];
+  const double fT = (c->rhohsrc - pn) / rcp;
+  for (int j = 0; j < n; ++j) {
+    double cs = 0.0;
+    for (int k = 0; k < N; ++k) cs += s.hk[k] * J[k * n + j];
+    J[N * n + j] = (j < N) ? (-cs - fT * s.cpk[j]) / rcp : ((-cs - scf) - fT * sdcp) / rcp;
   }
-  double fT = num / rcp;
-  for (int k = 0; k < N; ++k) {
-    for (int j = 0; j < N; ++j) J[k * n + j] = JC[k * n + j] * m->W[k] / m->W[j];
-    J[k * n + N] = JC[k * n + N] * m->W[k];
-  }
-  for (int j = 0; j < N; ++j) {
-    double s = 0.0;
-    for (int k = 0; k < N; ++k) s += hk[k] * J[k * n + j];
-    J[N * n + j] = (-s - fT * cpk[j]) / rcp;
-  }
-  double s = 0.0;
-  for (int k = 0; k < N; ++k) s += cpk[k] * fk[k] + hk[k] * J[k * n + N];
-  J[N * n + N] = (-s - fT * sdcp) / rcp;
   return 0;
 }
 

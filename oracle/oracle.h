@@ -72,6 +72,11 @@ void orc_mech_weights(int id, double* W);
 void orc_thermo(int id, double T, double* cpR, double* hRT, double* sR);
 /* rates of progress q[nr] (mol/cm^3/s) and molar production rates wdot[nsp]; C in mol/cm^3 */
 void orc_rates(int id, const double* C, double T, double* q, double* wdot);
+/* forward and reverse rates of progress per reaction (mol/cm^3/s) */
+void orc_rates_fr(int id, const double* C, double T, double* fwd, double* rev);
+/* the oracle's exp / log (the same routines the HIP kernels use) */
+double orc_exp(double x);
+double orc_log(double x);
 /* T from mass-specific enthalpy h (erg/g) and Y by Newton from Tguess; returns iterations */
 int orc_hy2t(int id, double h, const double* Y, double* T);
 

@@ -61,6 +61,11 @@ def lib():
         L.orc_react.argtypes = [C.c_int, C.c_long, dp, dp, dp, dp, dp, dp, ip, C.c_double,
                                 C.POINTER(BdfOpts), C.c_int, ip, C.POINTER(C.c_long)]
         L.orc_mech_weights.argtypes = [C.c_int, dp]
+        L.orc_rates_fr.argtypes = [C.c_int, dp, C.c_double, dp, dp]
+        L.orc_exp.argtypes = [C.c_double]
+        L.orc_exp.restype = C.c_double
+        L.orc_log.argtypes = [C.c_double]
+        L.orc_log.restype = C.c_double
         _LIB = L
     return _LIB
 
@@ -97,6 +102,12 @@ class Mech:
         q, w = np.zeros(self.nr), np.zeros(self.nsp)
         lib().orc_rates(self.id, _p(Cmol), float(T), _p(q), _p(w))
         return q, w
+
+    def rates_fr(self, Cmol, T):
+        Cmol = np.ascontiguousarray(Cmol, dtype=np.float64)
+        f, r = np.zeros(self.nr), np.zeros(self.nr)
+        lib().orc_rates_fr(self.id, _p(Cmol), float(T), _p(f), _p(r))
+        return f, r
 
     def hy2t(self, h, Y, Tguess):
         Y = np.ascontiguousarray(Y, dtype=np.float64)
@@ -148,6 +159,14 @@ class Mech:
         lib().orc_react(self.id, ncell, _p(rY), _p(rYsrc), _p(T), _p(rH), _p(rHsrc), _p(FC),
                         _ip(mask), float(dt), C.byref(o), int(nthreads), _ip(status), C.byref(nst))
         return dict(rY=rY, T=T, rH=rH, FC=FC, status=status, nst=nst.value)
+
+
+def exp(x):
+    return lib().orc_exp(float(x))
+
+
+def log(x):
+    return lib().orc_log(float(x))
 
 
 def lu_factor(A):

@@ -1,4 +1,4 @@
-// Persistent batched BDF reactor kernel for gfx950 (FP64 throughout).
+// Persistent batched BDF reactor kernel for gfx950 (FP64 throughout): device building blocks.
 //
 // Replaces the integration inside PelePhysics ReactorBase::react (call site
 // Source/PeleLMeX_Reactions.cpp:101-108): every unmasked cell of a box is advanced over
@@ -17,10 +17,12 @@
 //    Jacobian assembled row-by-row in a per-cell LDS matrix that is also the saved Jacobian.
 //  * the dense LU of M = I - gamma J uses partial pivoting without moving data: the pivot row
 //    of each step is broadcast across the group and each lane eliminates its own rows.
-//  * the integrator is one state machine per group with a single RHS call site per iteration,
-//    so the cells of a wavefront evaluate their right-hand sides together even when they are
-//    in different phases (initial step, Newton, restart); a finished group pulls the next cell
-//    from an atomic work queue, so stiff cells do not hold back their neighbours.
+//
+// Arithmetic: every expression is an explicit IEEE-754 tree (the library is built without
+// floating-point contraction; fma only where written), exp / log are this file's own
+// routines, and every sum over components is taken sequentially in component order through
+// an LDS buffer. Results therefore do not depend on G or on the launch shape, and they are
+// bit-identical to the CPU oracle, which evaluates the same trees.
 #ifndef PR_BDF_KERNEL_H
 #define PR_BDF_KERNEL_H
 #include <hip/hip_runtime.h>
@@ -63,17 +65,77 @@ struct KParams {
 
 enum { ST_OK = 0, ST_TOO_MUCH_WORK = -1, ST_ERR_FAILURE = -3, ST_CONV_FAILURE = -4, ST_GUARD = -99 };
 
+// ---------------------------------------------------------------- math (own exp / log)
+// exp: x = k ln2 + r (Cody-Waite, ln2 in two parts), degree-13 Taylor polynomial of e^r, 2^k.
+// log: x = 2^e m, m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1), log m = 2 s (1 + s^2/3 + ... + s^20/21).
+#define PRK_LN2_HI 6.93147180559945286227e-01
+#define PRK_LN2_LO 2.31904681384629955842e-17
+#define PRK_LOG2E 1.44269504088896338700e+00
+#define PRK_SQRT_HALF 7.07106781186547572737e-01
+#define PRK_LN10 2.30258509299404590109e+00
+#define PRK_INV_LN10 4.34294481903251816668e-01
+
+__device__ __forceinline__ double pr_exp(double x) {
+  if (x != x) return x;
+  if (x > 709.782712893384) return __builtin_huge_val();
+  if (x < -745.1332191019412) return 0.0;
+  const double k = rint(x * PRK_LOG2E);
+  double r = fma(-k, PRK_LN2_HI, x);
+  r = fma(-k, PRK_LN2_LO, r);
+  double p = 1.0 / 6227020800.0;
+  p = fma(p, r, 1.0 / 479001600.0);
+  p = fma(p, r, 1.0 / 39916800.0);
+  p = fma(p, r, 1.0 / 3628800.0);
+  p = fma(p, r, 1.0 / 362880.0);
+  p = fma(p, r, 1.0 / 40320.0);
+  p = fma(p, r, 1.0 / 5040.0);
+  p = fma(p, r, 1.0 / 720.0);
+  p = fma(p, r, 1.0 / 120.0);
+  p = fma(p, r, 1.0 / 24.0);
+  p = fma(p, r, 1.0 / 6.0);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return ldexp(p, (int)k);
+}
+
+__device__ __forceinline__ double pr_log(double x) {
+  if (!(x > 0.0)) return (x == 0.0) ? -__builtin_huge_val() : __builtin_nan("");
+  if (x == __builtin_huge_val()) return x;
+  int e;
+  double m = frexp(x, &e);
+  if (m < PRK_SQRT_HALF) {
+    m = m * 2.0;
+    e = e - 1;
+  }
+  const double f = m - 1.0;
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  double p = 1.0 / 21.0;
+  p = fma(p, z, 1.0 / 19.0);
+  p = fma(p, z, 1.0 / 17.0);
+  p = fma(p, z, 1.0 / 15.0);
+  p = fma(p, z, 1.0 / 13.0);
+  p = fma(p, z, 1.0 / 11.0);
+  p = fma(p, z, 1.0 / 9.0);
+  p = fma(p, z, 1.0 / 7.0);
+  p = fma(p, z, 1.0 / 5.0);
+  p = fma(p, z, 1.0 / 3.0);
+  const double t = z * p;
+  const double s2 = s + s;
+  const double logm = fma(s2, t, s2);
+  const double de = (double)e;
+  return fma(de, PRK_LN2_HI, fma(de, PRK_LN2_LO, logm));
+}
+
+__device__ __forceinline__ double pr_log10(double x) { return pr_log(x) * PRK_INV_LN10; }
+__device__ __forceinline__ double pr_pow(double x, double y) { return pr_exp(y * pr_log(x)); }
+
+// ---------------------------------------------------------------- group primitives
 __device__ __forceinline__ void group_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-template <int G>
-__device__ __forceinline__ double gsum(double v) {
-#pragma unroll
-  for (int m = G / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, G);
-  return v;
 }
 
 template <int G>
@@ -93,37 +155,78 @@ __device__ __forceinline__ void gargmax(double& v, int& idx) {
   }
 }
 
-// Select element `s` (runtime, < RPL) of a register array without dynamic indexing.
-template <int RPL>
-__device__ __forceinline__ double rsel(const double (&a)[RPL], int s) {
-  double v = a[0];
+template <int G>
+__device__ __forceinline__ double gmax(double v) {
 #pragma unroll
-  for (int t = 1; t < RPL; ++t) v = (s == t) ? a[t] : v;
+  for (int m = G / 2; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, G));
   return v;
 }
 
-__device__ __forceinline__ void nasa_eval(const SpRec& sp, double T, double lnT, double& cpR,
-                                          double& hRT, double& sR, double& dcpR) {
-  const double* a = (T < sp.nasa[0]) ? sp.nasa + 8 : sp.nasa + 1;
-  double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
-  cpR = a[0] + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4;
-  hRT = a[0] + a[1] * T / 2 + a[2] * T2 / 3 + a[3] * T3 / 4 + a[4] * T4 / 5 + a[5] / T;
-  sR = a[0] * lnT + a[1] * T + a[2] * T2 / 2 + a[3] * T3 / 3 + a[4] * T4 / 4 + a[6];
-  dcpR = a[1] + 2 * a[2] * T + 3 * a[3] * T2 + 4 * a[4] * T3;
+// Sums over components 0..CNT-1 of per-row values (one or two series), in component order,
+// returned to every lane of the group. `red` is the group's LDS buffer (2 x (N+1) doubles).
+template <int G, int RPL, int CNT, int NB>
+__device__ __forceinline__ void gseq(double* red, int lane, const double (&a)[RPL], const double (&b)[RPL],
+                                     double& sa, double& sb) {
+#pragma unroll
+  for (int s = 0; s < RPL; ++s) {
+    const int i = lane + G * s;
+    if (i < CNT) {
+      red[i] = a[s];
+      if (NB > 1) red[CNT + i] = b[s];
+    }
+  }
+  group_sync();
+  double x = 0.0, y = 0.0;
+#pragma unroll
+  for (int i = 0; i < CNT; ++i) {
+    x += red[i];
+    if (NB > 1) y += red[CNT + i];
+  }
+  sa = x;
+  sb = y;
+  group_sync();
 }
 
-__device__ __forceinline__ void nasa_hcp(const SpRec& sp, double T, double& cpR, double& hRT) {
-  const double* a = (T < sp.nasa[0]) ? sp.nasa + 8 : sp.nasa + 1;
-  double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
-  cpR = a[0] + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4;
-  hRT = a[0] + a[1] * T / 2 + a[2] * T2 / 3 + a[3] * T3 / 4 + a[4] * T4 / 5 + a[5] / T;
+template <int G, int RPL, int CNT>
+__device__ __forceinline__ double gseq1(double* red, int lane, const double (&a)[RPL]) {
+  double x, y;
+  gseq<G, RPL, CNT, 1>(red, lane, a, a, x, y);
+  return x;
+}
+
+template <int G, int RPL, int CNT>
+__device__ __forceinline__ void gseq2(double* red, int lane, const double (&a)[RPL], const double (&b)[RPL],
+                                      double& sa, double& sb) {
+  gseq<G, RPL, CNT, 2>(red, lane, a, b, sa, sb);
+}
+
+// ---------------------------------------------------------------- thermo and rates
+__device__ __forceinline__ void nasa_hcp(const SpRec& sp, double T, double invT, double& cpR, double& hRT) {
+  const int r = (T < sp.tmid) ? 0 : 1;
+  const double* c = sp.cp[r];
+  const double* h = sp.h[r];
+  cpR = fma(fma(fma(fma(c[4], T, c[3]), T, c[2]), T, c[1]), T, c[0]);
+  const double hp = fma(fma(fma(fma(h[4], T, h[3]), T, h[2]), T, h[1]), T, h[0]);
+  hRT = fma(h[5], invT, hp);
+}
+
+__device__ __forceinline__ void nasa_full(const SpRec& sp, double T, double invT, double lnT, double& cpR,
+                                          double& hRT, double& gRT, double& dcpR) {
+  const int r = (T < sp.tmid) ? 0 : 1;
+  nasa_hcp(sp, T, invT, cpR, hRT);
+  const double* s = sp.s[r];
+  const double* d = sp.dcp[r];
+  const double spv = fma(fma(fma(s[4], T, s[3]), T, s[2]), T, s[1]) * T;
+  const double sR = fma(s[0], lnT, spv + s[5]);
+  gRT = hRT - sR;
+  dcpR = fma(fma(fma(d[3], T, d[2]), T, d[1]), T, d[0]);
 }
 
 struct RxOut {
   double kf, kr, fwd, rev, dqdM, dqdT;
 };
 
-// One reaction at (C, T): effective rate coefficients, forward/reverse rates and, when JAC,
+// One reaction at (C, T): effective rate coefficients, forward / reverse rates and, when JAC,
 // d q / d[M] and d q / dT at constant concentrations.
 template <bool JAC>
 __device__ __forceinline__ void rx_eval(const RxRec& rx, const double* sC, const double* sG,
@@ -133,16 +236,16 @@ __device__ __forceinline__ void rx_eval(const RxRec& rx, const double* sC, const
   const int info = rx.info;
   const int type = info & 15, rev = (info >> 4) & 15, nre = (info >> 8) & 15, npr = (info >> 12) & 15;
   const int ntroe = (info >> 16) & 15;
-  double kinf = exp(rx.lnA + rx.beta * lnT - rx.Ta * invT);
-  double dlnkinf = (rx.beta + rx.Ta * invT) * invT;
+  const double kinf = pr_exp(fma(-rx.Ta, invT, fma(rx.beta, lnT, rx.lnA)));
+  const double dlnkinf = fma(rx.Ta, invT, rx.beta) * invT;
   double M = 0.0;
   if (type != PR_RX_ELEM) {
     M = rx.effd * Ctot;
-    int eb = rx.eff & 0xFFFF, ee = (rx.eff >> 16) & 0xFFFF;
-    for (int e = eb; e < ee; ++e) M += effval[e] * sC[effsp[e]];
+    const int eb = rx.eff & 0xFFFF, ee = (rx.eff >> 16) & 0xFFFF;
+    for (int e = eb; e < ee; ++e) M = fma(effval[e], sC[effsp[e]], M);
   }
-  int r0 = rx.reac & 0xFF, r1 = (rx.reac >> 8) & 0xFF, r2 = (rx.reac >> 16) & 0xFF;
-  int p0 = rx.prod & 0xFF, p1 = (rx.prod >> 8) & 0xFF, p2 = (rx.prod >> 16) & 0xFF;
+  const int r0 = rx.reac & 0xFF, r1 = (rx.reac >> 8) & 0xFF, r2 = (rx.reac >> 16) & 0xFF;
+  const int p0 = rx.prod & 0xFF, p1 = (rx.prod >> 8) & 0xFF, p2 = (rx.prod >> 16) & 0xFF;
   double pr = sC[r0];
   if (nre > 1) pr *= sC[r1];
   if (nre > 2) pr *= sC[r2];
@@ -152,80 +255,84 @@ __device__ __forceinline__ void rx_eval(const RxRec& rx, const double* sC, const
   double invKc = 0.0, dH = 0.0;
   const int sumnu = npr - nre;
   if (rev) {
-    double dG = sG[p0] - sG[r0];
+    double dG = sG[p0];
     if (npr > 1) dG += sG[p1];
     if (npr > 2) dG += sG[p2];
+    dG -= sG[r0];
     if (nre > 1) dG -= sG[r1];
     if (nre > 2) dG -= sG[r2];
-    invKc = exp(dG + sumnu * lnRTP);
+    invKc = pr_exp(fma((double)sumnu, lnRTP, dG));
     if (JAC) {
-      dH = sH[p0] - sH[r0];
+      dH = sH[p0];
       if (npr > 1) dH += sH[p1];
       if (npr > 2) dH += sH[p2];
+      dH -= sH[r0];
       if (nre > 1) dH -= sH[r1];
       if (nre > 2) dH -= sH[r2];
     }
   }
-  const double dlnKc = (dH - sumnu) * invT;
+  const double dlnKc = (dH - (double)sumnu) * invT;
   if (type <= PR_RX_THIRDBODY) {
-    double kr0 = kinf * invKc;
-    double fwd0 = kinf * pr, rev0 = kr0 * pp;
+    const double kr0 = kinf * invKc;
+    const double fwd0 = kinf * pr, rev0 = kr0 * pp;
+    double dq0 = 0.0;
+    if (JAC) dq0 = fwd0 * dlnkinf - rev0 * (dlnkinf - dlnKc);
     if (type == PR_RX_ELEM) {
-      o.kf = kinf; o.kr = kr0; o.fwd = fwd0; o.rev = rev0;
-      if (JAC) { o.dqdM = 0.0; o.dqdT = fwd0 * dlnkinf - rev0 * (dlnkinf - dlnKc); }
+      o.kf = kinf; o.kr = kr0; o.fwd = fwd0; o.rev = rev0; o.dqdM = 0.0; o.dqdT = dq0;
     } else {
       o.kf = M * kinf; o.kr = M * kr0; o.fwd = M * fwd0; o.rev = M * rev0;
-      if (JAC) { o.dqdM = fwd0 - rev0; o.dqdT = M * (fwd0 * dlnkinf - rev0 * (dlnkinf - dlnKc)); }
+      o.dqdM = fwd0 - rev0; o.dqdT = M * dq0;
     }
     return;
   }
-  double k0 = exp(rx.lnA0 + rx.beta0 * lnT - rx.Ta0 * invT);
-  double Pr = k0 * M / kinf;
-  Pr = Pr < 1e-300 ? 1e-300 : Pr;
-  double F = 1.0, dlogF_dx = 0.0, dlogF_dT = 0.0;
-  const double LN10 = 2.302585092994045684;
-  double dlnPr = 0.0;
-  if (JAC) dlnPr = (rx.beta0 + rx.Ta0 * invT) * invT - dlnkinf;
+  const double k0 = pr_exp(fma(-rx.Ta0, invT, fma(rx.beta0, lnT, rx.lnA0)));
+  double Pr = (k0 * M) / kinf;
+  if (Pr < 1e-300) Pr = 1e-300;
+  double F = 1.0, dlogF_dx = 0.0, dlogF_dT = 0.0, dlnPr = 0.0;
+  if (JAC) dlnPr = fma(rx.Ta0, invT, rx.beta0) * invT - dlnkinf;
   if (type == PR_RX_TROE) {
-    double e3 = exp(-T / rx.troe[1]), e1 = exp(-T / rx.troe[2]);
+    const double e3 = pr_exp(-T / rx.troe[1]), e1 = pr_exp(-T / rx.troe[2]);
     double Fc = (1.0 - rx.troe[0]) * e3 + rx.troe[0] * e1;
     double dFc = 0.0;
-    if (JAC) dFc = -(1.0 - rx.troe[0]) / rx.troe[1] * e3 - rx.troe[0] / rx.troe[2] * e1;
+    if (JAC) dFc = -((1.0 - rx.troe[0]) / rx.troe[1]) * e3 - (rx.troe[0] / rx.troe[2]) * e1;
     if (ntroe == 4) {
-      double e2 = exp(-rx.troe[3] * invT);
+      const double e2 = pr_exp(-rx.troe[3] * invT);
       Fc += e2;
-      if (JAC) dFc += rx.troe[3] * invT * invT * e2;
+      if (JAC) dFc += ((rx.troe[3] * invT) * invT) * e2;
     }
-    Fc = Fc < 1e-300 ? 1e-300 : Fc;
-    double logFc = log10(Fc);
-    double c = -0.4 - 0.67 * logFc, nn = 0.75 - 1.27 * logFc;
-    double x = log10(Pr);
-    double xc = x + c, D = nn - 0.14 * xc, f1 = xc / D;
-    double den = 1.0 + f1 * f1;
-    F = exp(logFc / den * LN10);
+    if (Fc < 1e-300) Fc = 1e-300;
+    const double logFc = pr_log10(Fc);
+    const double c = -0.4 - 0.67 * logFc, nn = 0.75 - 1.27 * logFc;
+    const double x = pr_log10(Pr);
+    const double xc = x + c, D = nn - 0.14 * xc, f1 = xc / D;
+    const double den = 1.0 + f1 * f1;
+    F = pr_exp((logFc / den) * PRK_LN10);
     if (JAC) {
       dlogF_dx = -logFc * 2.0 * f1 * (nn / (D * D)) / (den * den);
-      double dlogFc = dFc / (Fc * LN10);
-      double dx = dlnPr / LN10, dc = -0.67 * dlogFc, dnn = -1.27 * dlogFc;
-      double dxc = dx + dc, dD = dnn - 0.14 * dxc;
-      double df1 = (dxc * D - xc * dD) / (D * D);
+      const double dlogFc = dFc / (Fc * PRK_LN10);
+      const double dx = dlnPr / PRK_LN10, dc = -0.67 * dlogFc, dnn = -1.27 * dlogFc;
+      const double dxc = dx + dc, dD = dnn - 0.14 * dxc;
+      const double df1 = (dxc * D - xc * dD) / (D * D);
       dlogF_dT = (dlogFc * den - logFc * 2.0 * f1 * df1) / (den * den);
     }
   }
-  double kf = kinf * Pr / (1.0 + Pr) * F;
-  double kr = kf * invKc;
+  const double kf = ((kinf * Pr) / (1.0 + Pr)) * F;
+  const double kr = kf * invKc;
   o.kf = kf;
   o.kr = kr;
   o.fwd = kf * pr;
   o.rev = kr * pp;
+  o.dqdM = 0.0;
+  o.dqdT = 0.0;
   if (JAC) {
-    double dlnkf = dlnkinf + dlnPr / (1.0 + Pr) + LN10 * dlogF_dT;
-    o.dqdM = k0 * F * (1.0 / ((1.0 + Pr) * (1.0 + Pr)) + dlogF_dx / (1.0 + Pr)) * (pr - pp * invKc);
+    const double dlnkf = dlnkinf + dlnPr / (1.0 + Pr) + PRK_LN10 * dlogF_dT;
+    const double onep = 1.0 + Pr;
+    o.dqdM = ((k0 * F) * (1.0 / (onep * onep) + dlogF_dx / onep)) * (pr - pp * invKc);
     o.dqdT = o.fwd * dlnkf - o.rev * (dlnkf - dlnKc);
   }
 }
 
-// Views of the LDS-staged mechanism and one group's scratch.
+// ---------------------------------------------------------------- LDS views
 template <int N, int R>
 struct Smem {
   const SpRec* sp;
@@ -240,10 +347,11 @@ struct Smem {
   double* sH;    // N h/RT
   double* sHm;   // N mass-specific h
   double* sCpm;  // N mass-specific cp
-  double* rec;   // 5 R: kf, kr, dqdM, dqdT, q
+  double* rec;   // 5 R: kf, kr, dqdM, dqdT per reaction, then q
+  double* red;   // 2 (N+1): reduction buffer
   int* perm;     // N+1 pivot rows
   static constexpr int n = N + 1;
-  static constexpr int scratch_doubles = n * n + 5 * N + 5 * R + (n + 1) / 2 + 1;
+  static constexpr int scratch_doubles = n * n + 5 * N + 5 * R + 2 * n + (n + 1) / 2 + 1;
 };
 
 template <int N, int R, int G>
@@ -253,65 +361,64 @@ struct Cell {
   static constexpr int RXL = (R + G - 1) / G;
 };
 
+// ---------------------------------------------------------------- right-hand side
 // Reactor right-hand side f(t, y) on the group; T from (h(t), Y) by Newton, started at y_T.
 template <int N, int R, int G>
-__device__ __forceinline__ void rhs_eval(const Smem<N, R>& S, int lane, double t, const double (&y)[Cell<N, R, G>::RPL],
-                         const double (&src)[Cell<N, R, G>::RPL], double rhoh0, double rhohsrc,
-                         double (&f)[Cell<N, R, G>::RPL]) {
+__device__ __forceinline__ void rhs_eval(const Smem<N, R>& S, int lane, double t,
+                                         const double (&y)[Cell<N, R, G>::RPL],
+                                         const double (&src)[Cell<N, R, G>::RPL], double rhoh0,
+                                         double rhohsrc, double (&f)[Cell<N, R, G>::RPL]) {
   constexpr int RPL = Cell<N, R, G>::RPL;
-  constexpr int n = N + 1;
-  double part = 0.0;
-#pragma unroll
-  for (int s = 0; s < RPL; ++s) {
-    int i = lane + G * s;
-    if (i < N) part += y[s];
-  }
-  const double rho = gsum<G>(part);
+  const double rho = gseq1<G, RPL, N>(S.red, lane, y);
   const double hmix = (rhoh0 + t * rhohsrc) / rho;
   double T = gbcast<G>(y[N / G], N % G);
-  const double RU = PR_RU;
   for (int it = 0; it < 20; ++it) {
-    double ph = 0.0, pc = 0.0;
+    const double invT = 1.0 / T;
+    double ph[RPL], pc[RPL];
 #pragma unroll
     for (int s = 0; s < RPL; ++s) {
-      int i = lane + G * s;
+      const int i = lane + G * s;
+      ph[s] = 0.0;
+      pc[s] = 0.0;
       if (i < N) {
+        const SpRec& sp = S.sp[i];
         double cpR, hRT;
-        nasa_hcp(S.sp[i], T, cpR, hRT);
-        double Y = y[s] / rho, RW = RU * S.sp[i].invW;
-        ph += Y * hRT * RW * T;
-        pc += Y * cpR * RW;
+        nasa_hcp(sp, T, invT, cpR, hRT);
+        const double Y = y[s] / rho;
+        ph[s] = ((Y * hRT) * sp.RW) * T;
+        pc[s] = (Y * cpR) * sp.RW;
       }
     }
-    double hm = gsum<G>(ph), cpm = gsum<G>(pc);
-    double dT = (hmix - hm) / cpm;
-    T += dT;
+    double hm, cpm;
+    gseq2<G, RPL, N>(S.red, lane, ph, pc, hm, cpm);
+    const double dT = (hmix - hm) / cpm;
+    T = T + dT;
     if (fabs(dT) <= 1e-10 * T) break;
   }
-  const double lnT = log(T), invT = 1.0 / T, lnRTP = log(RU * T / PR_PATM);
-  double hk[RPL], cpk[RPL], ctot = 0.0;
+  const double lnT = pr_log(T), invT = 1.0 / T, lnRTP = pr_log((PR_RU * T) / PR_PATM);
+  double hk[RPL], cpk[RPL], cc[RPL];
 #pragma unroll
   for (int s = 0; s < RPL; ++s) {
-    int i = lane + G * s;
+    const int i = lane + G * s;
     hk[s] = 0.0;
     cpk[s] = 0.0;
+    cc[s] = 0.0;
     if (i < N) {
-      double cpR, hRT, sR, dcp;
-      nasa_eval(S.sp[i], T, lnT, cpR, hRT, sR, dcp);
-      double RW = RU * S.sp[i].invW;
-      double C = y[s] * S.sp[i].invW;
+      const SpRec& sp = S.sp[i];
+      double cpR, hRT, gRT, dcp;
+      nasa_full(sp, T, invT, lnT, cpR, hRT, gRT, dcp);
+      const double C = y[s] * sp.invW;
       S.sC[i] = C;
-      S.sG[i] = hRT - sR;
-      ctot += C;
-      hk[s] = hRT * RW * T;
-      cpk[s] = cpR * RW;
+      S.sG[i] = gRT;
+      cc[s] = C;
+      hk[s] = (hRT * sp.RW) * T;
+      cpk[s] = cpR * sp.RW;
     }
   }
-  const double Ctot = gsum<G>(ctot);
-  group_sync();
+  const double Ctot = gseq1<G, RPL, N>(S.red, lane, cc);
 #pragma unroll 1
   for (int m = 0; m < Cell<N, R, G>::RXL; ++m) {
-    int r = lane + G * m;
+    const int r = lane + G * m;
     if (r < R) {
       RxOut o;
       rx_eval<false>(S.rx[r], S.sC, S.sG, S.sH, Ctot, T, lnT, invT, lnRTP, S.effsp, S.effval, o);
@@ -319,64 +426,65 @@ __device__ __forceinline__ void rhs_eval(const Smem<N, R>& S, int lane, double t
     }
   }
   group_sync();
-  double pn = 0.0, pcp = 0.0;
+  double pn[RPL], pcp[RPL];
 #pragma unroll
   for (int s = 0; s < RPL; ++s) {
-    int i = lane + G * s;
+    const int i = lane + G * s;
     f[s] = 0.0;
+    pn[s] = 0.0;
+    pcp[s] = 0.0;
     if (i < N) {
       double w = 0.0;
       for (int e = S.incs[i]; e < S.incs[i + 1]; ++e) {
-        int rn = S.inc[e];
+        const int rn = S.inc[e];
         w += (double)(rn >> 16) * S.rec[4 * R + (rn & 0xFFFF)];
       }
       f[s] = w * S.sp[i].W + src[s];
-      pn += hk[s] * f[s];
-      pcp += y[s] * cpk[s];
+      pn[s] = hk[s] * f[s];
+      pcp[s] = y[s] * cpk[s];
     }
   }
-  const double num = rhohsrc - gsum<G>(pn);
-  const double rcp = gsum<G>(pcp);
-  const double fT = num / rcp;
+  double sn, rcp;
+  gseq2<G, RPL, N>(S.red, lane, pn, pcp, sn, rcp);
+  const double fT = (rhohsrc - sn) / rcp;
   if (lane == N % G) f[N / G] = fT;
-  (void)n;
 }
 
-// Analytic Jacobian of the right-hand side with T = y_T, written into S.Js (row-major).
+// ---------------------------------------------------------------- analytic Jacobian
+// Exact Jacobian of the right-hand side with T = y_T, written into S.Js (row-major).
 template <int N, int R, int G>
 __device__ __forceinline__ void jac_eval(const Smem<N, R>& S, int lane, const double (&y)[Cell<N, R, G>::RPL],
-                         const double (&src)[Cell<N, R, G>::RPL], double rhohsrc) {
+                                         const double (&src)[Cell<N, R, G>::RPL], double rhohsrc) {
   constexpr int RPL = Cell<N, R, G>::RPL;
   constexpr int n = N + 1;
-  const double RU = PR_RU;
   const double T = gbcast<G>(y[N / G], N % G);
-  const double lnT = log(T), invT = 1.0 / T, lnRTP = log(RU * T / PR_PATM);
-  double ctot = 0.0, cpk[RPL], dcpk[RPL];
+  const double lnT = pr_log(T), invT = 1.0 / T, lnRTP = pr_log((PR_RU * T) / PR_PATM);
+  double cc[RPL], cpk[RPL], dcpk[RPL];
 #pragma unroll
   for (int s = 0; s < RPL; ++s) {
-    int i = lane + G * s;
+    const int i = lane + G * s;
+    cc[s] = 0.0;
     cpk[s] = 0.0;
     dcpk[s] = 0.0;
     if (i < N) {
-      double cpR, hRT, sR, dcp;
-      nasa_eval(S.sp[i], T, lnT, cpR, hRT, sR, dcp);
-      double RW = RU * S.sp[i].invW;
-      double C = y[s] * S.sp[i].invW;
+      const SpRec& sp = S.sp[i];
+      double cpR, hRT, gRT, dcp;
+      nasa_full(sp, T, invT, lnT, cpR, hRT, gRT, dcp);
+      const double C = y[s] * sp.invW;
       S.sC[i] = C;
-      S.sG[i] = hRT - sR;
+      S.sG[i] = gRT;
       S.sH[i] = hRT;
-      S.sHm[i] = hRT * RW * T;
-      S.sCpm[i] = cpR * RW;
-      cpk[s] = cpR * RW;
-      dcpk[s] = dcp * RW;
-      ctot += C;
+      S.sHm[i] = (hRT * sp.RW) * T;
+      S.sCpm[i] = cpR * sp.RW;
+      cc[s] = C;
+      cpk[s] = cpR * sp.RW;
+      dcpk[s] = dcp * sp.RW;
     }
   }
-  const double Ctot = gsum<G>(ctot);
-  group_sync();
+  const double Ctot = gseq1<G, RPL, N>(S.red, lane, cc);
 #pragma unroll 1
   for (int m = 0; m < Cell<N, R, G>::RXL; ++m) {
-    int r = lane + G * m;
+    const int r = lane + G * m;
     if (r < R) {
       RxOut o;
       rx_eval<true>(S.rx[r], S.sC, S.sG, S.sH, Ctot, T, lnT, invT, lnRTP, S.effsp, S.effval, o);
@@ -389,10 +497,14 @@ __device__ __forceinline__ void jac_eval(const Smem<N, R>& S, int lane, const do
     }
   }
   group_sync();
-  double pn = 0.0, prcp = 0.0, psd = 0.0, pcf = 0.0;
+  double pn[RPL], prcp[RPL], psd[RPL], pcf[RPL];
 #pragma unroll
   for (int s = 0; s < RPL; ++s) {
-    int k = lane + G * s;
+    const int k = lane + G * s;
+    pn[s] = 0.0;
+    prcp[s] = 0.0;
+    psd[s] = 0.0;
+    pcf[s] = 0.0;
     if (k < N) {
       double* row = S.Js + k * n;
       for (int j = 0; j < n; ++j) row[j] = 0.0;
@@ -405,66 +517,65 @@ __device__ __forceinline__ void jac_eval(const Smem<N, R>& S, int lane, const do
         const double* rc = S.rec + 4 * r;
         const int info = rx.info, type = info & 15, nre = (info >> 8) & 15, npr = (info >> 12) & 15;
         w += nu * S.rec[4 * R + r];
-        int a0 = rx.reac & 0xFF, a1 = (rx.reac >> 8) & 0xFF, a2 = (rx.reac >> 16) & 0xFF;
-        double c0 = S.sC[a0], c1 = nre > 1 ? S.sC[a1] : 1.0, c2 = nre > 2 ? S.sC[a2] : 1.0;
-        double kf = nu * rc[0];
-        row[a0] += kf * c1 * c2;
-        if (nre > 1) row[a1] += kf * c0 * c2;
-        if (nre > 2) row[a2] += kf * c0 * c1;
-        int b0 = rx.prod & 0xFF, b1 = (rx.prod >> 8) & 0xFF, b2 = (rx.prod >> 16) & 0xFF;
-        double d0 = S.sC[b0], d1 = npr > 1 ? S.sC[b1] : 1.0, d2 = npr > 2 ? S.sC[b2] : 1.0;
-        double kr = nu * rc[1];
-        row[b0] -= kr * d1 * d2;
-        if (npr > 1) row[b1] -= kr * d0 * d2;
-        if (npr > 2) row[b2] -= kr * d0 * d1;
+        const int a0 = rx.reac & 0xFF, a1 = (rx.reac >> 8) & 0xFF, a2 = (rx.reac >> 16) & 0xFF;
+        const double c0 = S.sC[a0], c1 = nre > 1 ? S.sC[a1] : 1.0, c2 = nre > 2 ? S.sC[a2] : 1.0;
+        const double kf = nu * rc[0];
+        row[a0] += (kf * c1) * c2;
+        if (nre > 1) row[a1] += (kf * c0) * c2;
+        if (nre > 2) row[a2] += (kf * c0) * c1;
+        const int b0 = rx.prod & 0xFF, b1 = (rx.prod >> 8) & 0xFF, b2 = (rx.prod >> 16) & 0xFF;
+        const double d0 = S.sC[b0], d1 = npr > 1 ? S.sC[b1] : 1.0, d2 = npr > 2 ? S.sC[b2] : 1.0;
+        const double kr = nu * rc[1];
+        row[b0] -= (kr * d1) * d2;
+        if (npr > 1) row[b1] -= (kr * d0) * d2;
+        if (npr > 2) row[b2] -= (kr * d0) * d1;
         if (type != PR_RX_ELEM) {
-          double dm = nu * rc[2];
+          const double dm = nu * rc[2];
           Sd += dm * rx.effd;
-          int eb = rx.eff & 0xFFFF, ee = (rx.eff >> 16) & 0xFFFF;
+          const int eb = rx.eff & 0xFFFF, ee = (rx.eff >> 16) & 0xFFFF;
           for (int q = eb; q < ee; ++q) row[S.effsp[q]] += dm * S.effval[q];
         }
         JT += nu * rc[3];
       }
       const double Wk = S.sp[k].W;
-      for (int j = 0; j < N; ++j) row[j] = (row[j] + Sd) * Wk * S.sp[j].invW;
+      for (int j = 0; j < N; ++j) row[j] = ((row[j] + Sd) * Wk) * S.sp[j].invW;
       row[N] = JT * Wk;
-      double fk = w * Wk + src[s];
-      pn += S.sHm[k] * fk;
-      prcp += y[s] * cpk[s];
-      psd += y[s] * dcpk[s];
-      pcf += cpk[s] * fk;
+      const double fk = w * Wk + src[s];
+      pn[s] = S.sHm[k] * fk;
+      prcp[s] = y[s] * cpk[s];
+      psd[s] = y[s] * dcpk[s];
+      pcf[s] = cpk[s] * fk;
     }
   }
-  const double num = rhohsrc - gsum<G>(pn);
-  const double rcp = gsum<G>(prcp);
-  const double sdcp = gsum<G>(psd);
-  const double scf = gsum<G>(pcf);
-  const double fT = num / rcp;
-  group_sync();
+  double sn, rcp, sdcp, scf;
+  gseq2<G, RPL, N>(S.red, lane, pn, prcp, sn, rcp);
+  gseq2<G, RPL, N>(S.red, lane, psd, pcf, sdcp, scf);
+  const double fT = (rhohsrc - sn) / rcp;
 #pragma unroll
   for (int s = 0; s < RPL; ++s) {
-    int j = lane + G * s;
+    const int j = lane + G * s;
     if (j < n) {
       double cs = 0.0;
       for (int k = 0; k < N; ++k) cs += S.sHm[k] * S.Js[k * n + j];
-      S.Js[N * n + j] = (j < N) ? (-cs - fT * S.sCpm[j]) / rcp : (-cs - scf - fT * sdcp) / rcp;
+      S.Js[N * n + j] = (j < N) ? (-cs - fT * S.sCpm[j]) / rcp : ((-cs - scf) - fT * sdcp) / rcp;
     }
   }
   group_sync();
 }
 
+// ---------------------------------------------------------------- dense LU and solve
 // LU of M = I - gamma * Js with partial pivoting (first maximal |pivot| in row order).
 // Returns true when a zero pivot is met. Rows stay in the owning lanes.
 template <int N, int R, int G>
 __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double gamma,
-                          double (&lu)[Cell<N, R, G>::RPL][N + 1], int (&pst)[Cell<N, R, G>::RPL],
-                          double (&rinv)[Cell<N, R, G>::RPL]) {
+                                          double (&lu)[Cell<N, R, G>::RPL][N + 1],
+                                          int (&pst)[Cell<N, R, G>::RPL], double (&rinv)[Cell<N, R, G>::RPL]) {
   constexpr int RPL = Cell<N, R, G>::RPL;
   constexpr int n = N + 1;
   constexpr int UNPIV = 1 << 20;
 #pragma unroll
   for (int s = 0; s < RPL; ++s) {
-    int i = lane + G * s;
+    const int i = lane + G * s;
     pst[s] = (i < n) ? UNPIV : -1;
     rinv[s] = 0.0;
 #pragma unroll
@@ -516,7 +627,7 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
       pj = gbcast<G>(pj, owner);
 #pragma unroll
       for (int s = 0; s < RPL; ++s)
-        if (pst[s] > k) lu[s][j] -= lu[s][k] * pj;
+        if (pst[s] > k) lu[s][j] = fma(-lu[s][k], pj, lu[s][j]);
     }
   }
   group_sync();
@@ -526,8 +637,8 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
 // Solve M x = b with the factors from lu_factor; b is distributed by row, x by component.
 template <int N, int R, int G>
 __device__ __forceinline__ void lu_solve(const Smem<N, R>& S, int lane, const double (&lu)[Cell<N, R, G>::RPL][N + 1],
-                         const int (&pst)[Cell<N, R, G>::RPL], const double (&rinv)[Cell<N, R, G>::RPL],
-                         double (&b)[Cell<N, R, G>::RPL], double (&x)[Cell<N, R, G>::RPL]) {
+                                         const int (&pst)[Cell<N, R, G>::RPL], const double (&rinv)[Cell<N, R, G>::RPL],
+                                         double (&b)[Cell<N, R, G>::RPL], double (&x)[Cell<N, R, G>::RPL]) {
   constexpr int RPL = Cell<N, R, G>::RPL;
   constexpr int n = N + 1;
 #pragma unroll
@@ -540,7 +651,7 @@ __device__ __forceinline__ void lu_solve(const Smem<N, R>& S, int lane, const do
     const double yk = gbcast<G>(v, owner);
 #pragma unroll
     for (int s = 0; s < RPL; ++s)
-      if (pst[s] > k) b[s] -= lu[s][k] * yk;
+      if (pst[s] > k) b[s] = fma(-lu[s][k], yk, b[s]);
   }
 #pragma unroll
   for (int s = 0; s < RPL; ++s) x[s] = 0.0;
@@ -554,25 +665,24 @@ __device__ __forceinline__ void lu_solve(const Smem<N, R>& S, int lane, const do
     const double xk = gbcast<G>(v, owner);
 #pragma unroll
     for (int s = 0; s < RPL; ++s)
-      if (pst[s] >= 0 && pst[s] < k) b[s] -= lu[s][k] * xk;
+      if (pst[s] >= 0 && pst[s] < k) b[s] = fma(-lu[s][k], xk, b[s]);
     if (lane == k % G) x[k / G] = xk;
   }
 }
 
+// Weighted RMS norm over the n components, summed in component order.
 template <int N, int R, int G>
-__device__ __forceinline__ double wrms(int lane, const double (&v)[Cell<N, R, G>::RPL],
+__device__ __forceinline__ double wrms(const Smem<N, R>& S, int lane, const double (&v)[Cell<N, R, G>::RPL],
                                        const double (&w)[Cell<N, R, G>::RPL]) {
   constexpr int RPL = Cell<N, R, G>::RPL;
-  double p = 0.0;
+  double sq[RPL];
 #pragma unroll
   for (int s = 0; s < RPL; ++s) {
-    int i = lane + G * s;
-    if (i < N + 1) {
-      double t = v[s] * w[s];
-      p += t * t;
-    }
+    const double t = v[s] * w[s];
+    sq[s] = t * t;
   }
-  return sqrt(gsum<G>(p) / (N + 1));
+  const double ss = gseq1<G, RPL, N + 1>(S.red, lane, sq);
+  return sqrt(ss / (N + 1));
 }
 
 }  // namespace prk

@@ -1,6 +1,6 @@
 // Packed mechanism tables as the HIP kernels read them (host packs, device stages into LDS).
 //
-// One reaction record is 12 doubles + 4 ints; one species record is 17 doubles. The host
+// One reaction record is 12 doubles + 4 ints; one species record is 46 doubles. The host
 // builds the blob once per reactor from the generated pr_mech_t tables (pr_create) and the
 // kernels copy it into LDS at launch, so every Arrhenius / third-body / falloff parameter the
 // rate and Jacobian code touches is an LDS read.
@@ -26,9 +26,12 @@ struct RxRec {
   int32_t eff;                // eff_begin | eff_end << 16
 };
 
+// NASA-7 fits rewritten for Horner evaluation (range 0 = low, T < T_mid; 1 = high):
+//   cp/R = cp[0..4] in T;  h/RT = h[0..4] in T + h[5] / T;  s/R = s[0] ln T + T (s[1..4] in T) + s[5];
+//   d(cp/R)/dT = dcp[0..3] in T.  RW = R_u / W, invW = 1 / W (correctly rounded on the host).
 struct SpRec {
-  double nasa[15];            // T_mid, high a1..a7, low a1..a7
-  double W, invW;
+  double cp[2][5], h[2][6], s[2][6], dcp[2][4];
+  double W, invW, RW, tmid;
 };
 
 // Blob layout (byte offsets, 16-byte aligned sections):

@@ -72,7 +72,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
   S.sHm = S.sH + N;
   S.sCpm = S.sHm + N;
   S.rec = S.sCpm + N;
-  S.perm = reinterpret_cast<int*>(S.rec + 5 * R);
+  S.red = S.rec + 5 * R;
+  S.perm = reinterpret_cast<int*>(S.red + 2 * n);
 
   const double tout = P.dt;
   const int qmax = P.max_order < 1 ? 1 : (P.max_order > Q ? Q : P.max_order);
@@ -418,8 +419,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
           hinv = r > hinv ? r : hinv;
         }
       }
-#pragma unroll
-      for (int m = G / 2; m >= 1; m >>= 1) hinv = fmax(hinv, __shfl_xor(hinv, m, G));
+      hinv = gmax<G>(hinv);
       const double tround = DBL_EPSILON * fabs(tout);
       const double hlb = 100.0 * tround;
       double hub = 0.1 * fabs(tout);
@@ -443,7 +443,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
       double tmp[RPL];
 #pragma unroll
       for (int s = 0; s < RPL; ++s) tmp[s] = (f[s] - zn[1][s]) / hg;
-      const double yddnrm = wrms<N, R, G>(lane, tmp, ewt);
+      const double yddnrm = wrms<N, R, G>(S, lane, tmp, ewt);
       double hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
       bool done = false;
       if (c.count1 == 4) {
@@ -520,12 +520,12 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
         }
 #pragma unroll
         for (int s = 0; s < RPL; ++s) acor[s] += dlt[s];
-        const double del = wrms<N, R, G>(lane, dlt, ewt);
+        const double del = wrms<N, R, G>(S, lane, dlt, ewt);
         const int mnewt = c.mnewt;
         if (mnewt > 0) c.crate = fmax(0.3 * c.crate, del / c.delp);
         const double dcon = del * fmin(1.0, c.crate) / c.tq[4];
         if (dcon <= 1.0) {
-          c.acnrm = (mnewt == 0) ? del : wrms<N, R, G>(lane, acor, ewt);
+          c.acnrm = (mnewt == 0) ? del : wrms<N, R, G>(S, lane, acor, ewt);
           c.jcur = 0;
           outcome = 1;
         } else if (mnewt >= 1 && del > 2.0 * c.delp) {
@@ -599,7 +599,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
             c.eta = 1.0;
           } else {
             const int L = c.L;
-            const double etaq = 1.0 / (pow(6.0 * dsm, 1.0 / L) + 1e-6);
+            const double etaq = 1.0 / (pr_pow(6.0 * dsm, 1.0 / L) + 1e-6);
             if (c.qwait != 0) {
               c.eta = etaq;
               c.qprime = q;
@@ -615,11 +615,11 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
                   if (j == q)
 #pragma unroll
                     for (int s = 0; s < RPL; ++s) zq[s] = zn[j][s];
-                const double ddn = wrms<N, R, G>(lane, zq, ewt) * c.tq[1];
-                etaqm1 = 1.0 / (pow(6.0 * ddn, 1.0 / q) + 1e-6);
+                const double ddn = wrms<N, R, G>(S, lane, zq, ewt) * c.tq[1];
+                etaqm1 = 1.0 / (pr_pow(6.0 * ddn, 1.0 / q) + 1e-6);
               }
               if (q != qmax && c.saved_tq5 != 0.0) {
-                const double cquot = (c.tq[5] / c.saved_tq5) * pow(c.h / c.tau[2], (double)L);
+                const double cquot = (c.tq[5] / c.saved_tq5) * pr_pow(c.h / c.tau[2], (double)L);
                 double tmp[RPL];
 #pragma unroll
                 for (int s = 0; s < RPL; ++s) tmp[s] = acor[s];
@@ -628,8 +628,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
                   if (j == qmax)
 #pragma unroll
                     for (int s = 0; s < RPL; ++s) tmp[s] = acor[s] - cquot * zn[j][s];
-                const double dup = wrms<N, R, G>(lane, tmp, ewt) * c.tq[3];
-                etaqp1 = 1.0 / (pow(10.0 * dup, 1.0 / (L + 1)) + 1e-6);
+                const double dup = wrms<N, R, G>(S, lane, tmp, ewt) * c.tq[3];
+                etaqp1 = 1.0 / (pr_pow(10.0 * dup, 1.0 / (L + 1)) + 1e-6);
               }
               const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
               if (etam < 1.5) {
@@ -693,7 +693,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
           } else {
             c.etamax = 1.0;
             if (c.nef <= 3) {
-              double eta = 1.0 / (pow(6.0 * dsm, 1.0 / c.L) + 1e-6);
+              double eta = 1.0 / (pr_pow(6.0 * dsm, 1.0 / c.L) + 1e-6);
               eta = fmax(0.1, eta);
               if (c.nef >= 2) eta = fmin(eta, 0.2);
               c.eta = eta;
@@ -734,6 +734,65 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
     atomicAdd(&P.stats[6], (unsigned long long)c.s_netf);
     atomicAdd(&P.stats[7], (unsigned long long)c.s_ncfn);
     if (c.s_fail) atomicAdd(P.nfail, c.s_fail);
+  }
+}
+
+
+// One group per cell: evaluates the reactor right-hand side (MODE 0) or the analytic Jacobian
+// (MODE 1) for a batch of states stored component-major (y[i * ncells + c]).
+template <int N, int R, int G, int MODE>
+__global__ void __launch_bounds__(BLOCK) eval_kernel(const unsigned char* blob, int blob_bytes, int neff, int ninc,
+                                                     long ncells, const double* y, const double* rYsrc,
+                                                     const double* rhoh0, const double* rhohsrc, double t,
+                                                     double* out) {
+  constexpr int n = N + 1;
+  constexpr int RPL = Cell<N, R, G>::RPL;
+  extern __shared__ __align__(16) unsigned char smem[];
+  {
+    const uint4* srcb = reinterpret_cast<const uint4*>(blob);
+    uint4* dst = reinterpret_cast<uint4*>(smem);
+    for (int b = threadIdx.x; b < blob_bytes / 16; b += blockDim.x) dst[b] = srcb[b];
+  }
+  __syncthreads();
+  const BlobLayout BL = blob_layout(N, R, neff, ninc);
+  Smem<N, R> S;
+  S.sp = reinterpret_cast<const SpRec*>(smem + BL.off_sp);
+  S.rx = reinterpret_cast<const RxRec*>(smem + BL.off_rx);
+  S.effsp = reinterpret_cast<const int*>(smem + BL.off_effsp);
+  S.effval = reinterpret_cast<const double*>(smem + BL.off_effval);
+  S.incs = reinterpret_cast<const int*>(smem + BL.off_incs);
+  S.inc = reinterpret_cast<const int*>(smem + BL.off_inc);
+  const int lane = threadIdx.x % G;
+  const int grp = threadIdx.x / G;
+  double* scr = reinterpret_cast<double*>(smem + BL.bytes) + (long)grp * group_doubles<N, R>();
+  S.Js = scr + CTL_DOUBLES;
+  S.sC = S.Js + n * n;
+  S.sG = S.sC + N;
+  S.sH = S.sG + N;
+  S.sHm = S.sH + N;
+  S.sCpm = S.sHm + N;
+  S.rec = S.sCpm + N;
+  S.red = S.rec + 5 * R;
+  S.perm = reinterpret_cast<int*>(S.red + 2 * n);
+  const long c = (long)blockIdx.x * (blockDim.x / G) + grp;
+  if (c >= ncells) return;  // whole groups leave together
+  double yv[RPL], src[RPL], f[RPL];
+#pragma unroll
+  for (int s = 0; s < RPL; ++s) {
+    const int i = lane + G * s;
+    yv[s] = (i < n) ? y[(long)i * ncells + c] : 0.0;
+    src[s] = (i < N) ? rYsrc[(long)i * ncells + c] : 0.0;
+  }
+  if (MODE == 0) {
+    rhs_eval<N, R, G>(S, lane, t, yv, src, rhoh0[c], rhohsrc[c], f);
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      const int i = lane + G * s;
+      if (i < n) out[(long)i * ncells + c] = f[s];
+    }
+  } else {
+    jac_eval<N, R, G>(S, lane, yv, src, rhohsrc[c]);
+    for (int e = lane; e < n * n; e += G) out[(long)e * ncells + c] = S.Js[e];
   }
 }
 

@@ -49,11 +49,29 @@ const pr_mech_t* find_mech(const char* name) {
 }
 
 using Launch = hipError_t (*)(const prk::KParams&, int blocks, int block, size_t shmem, hipStream_t);
+using EvalLaunch = hipError_t (*)(int mode, const unsigned char* blob, int blob_bytes, int neff, int ninc, long ncells,
+                                  const double* y, const double* rYsrc, const double* rhoh0, const double* rhohsrc,
+                                  double t, double* out, int block, size_t shmem, hipStream_t st);
 using Occ = hipError_t (*)(int* nb, int block, size_t shmem);
 
 template <int N, int R, int G>
 hipError_t launch_t(const prk::KParams& P, int blocks, int block, size_t shmem, hipStream_t st) {
   hipLaunchKernelGGL((prk::react_kernel<N, R, G>), dim3(blocks), dim3(block), shmem, st, P);
+  return hipGetLastError();
+}
+
+template <int N, int R, int G>
+hipError_t eval_t(int mode, const unsigned char* blob, int blob_bytes, int neff, int ninc, long ncells, const double* y,
+                  const double* rYsrc, const double* rhoh0, const double* rhohsrc, double t, double* out, int block,
+                  size_t shmem, hipStream_t st) {
+  const long groups = block / G;
+  const int blocks = (int)((ncells + groups - 1) / groups);
+  if (mode == 0)
+    hipLaunchKernelGGL((prk::eval_kernel<N, R, G, 0>), dim3(blocks), dim3(block), shmem, st, blob, blob_bytes, neff,
+                       ninc, ncells, y, rYsrc, rhoh0, rhohsrc, t, out);
+  else
+    hipLaunchKernelGGL((prk::eval_kernel<N, R, G, 1>), dim3(blocks), dim3(block), shmem, st, blob, blob_bytes, neff,
+                       ninc, ncells, y, rYsrc, rhoh0, rhohsrc, t, out);
   return hipGetLastError();
 }
 
@@ -72,12 +90,13 @@ struct Variant {
   const char* mech;
   int G;
   Launch launch;
+  EvalLaunch eval;
   Occ occ;
   int scratch;  // bytes per cell group
 };
 
 #define PR_VARIANT(NAME, NS, NR, G) \
-  {NAME, G, &launch_t<NS, NR, G>, &occ_t<NS, NR, G>, scratch_bytes<NS, NR>()}
+  {NAME, G, &launch_t<NS, NR, G>, &eval_t<NS, NR, G>, &occ_t<NS, NR, G>, scratch_bytes<NS, NR>()}
 
 const Variant kVariants[] = {
     PR_VARIANT("LiDryer", PR_LIDRYER_NSP, PR_LIDRYER_NR, 16),
@@ -98,9 +117,31 @@ std::vector<unsigned char> pack_blob(const pr_mech_t* m) {
   std::vector<unsigned char> b(L.bytes, 0);
   auto* sp = reinterpret_cast<prk::SpRec*>(b.data() + L.off_sp);
   for (int k = 0; k < m->nsp; ++k) {
-    for (int c = 0; c < 15; ++c) sp[k].nasa[c] = m->nasa[15 * k + c];
-    sp[k].W = m->W[k];
-    sp[k].invW = 1.0 / m->W[k];
+    prk::SpRec& o = sp[k];
+    for (int rg = 0; rg < 2; ++rg) {
+      const double* a = m->nasa + 15 * k + (rg == 0 ? 8 : 1);
+      for (int c = 0; c < 5; ++c) o.cp[rg][c] = a[c];
+      o.h[rg][0] = a[0];
+      o.h[rg][1] = a[1] / 2.0;
+      o.h[rg][2] = a[2] / 3.0;
+      o.h[rg][3] = a[3] / 4.0;
+      o.h[rg][4] = a[4] / 5.0;
+      o.h[rg][5] = a[5];
+      o.s[rg][0] = a[0];
+      o.s[rg][1] = a[1];
+      o.s[rg][2] = a[2] / 2.0;
+      o.s[rg][3] = a[3] / 3.0;
+      o.s[rg][4] = a[4] / 4.0;
+      o.s[rg][5] = a[6];
+      o.dcp[rg][0] = a[1];
+      o.dcp[rg][1] = 2.0 * a[2];
+      o.dcp[rg][2] = 3.0 * a[3];
+      o.dcp[rg][3] = 4.0 * a[4];
+    }
+    o.W = m->W[k];
+    o.invW = 1.0 / m->W[k];
+    o.RW = PR_RU / m->W[k];
+    o.tmid = m->nasa[15 * k];
   }
   auto* rx = reinterpret_cast<prk::RxRec*>(b.data() + L.off_rx);
   for (int r = 0; r < m->nr; ++r) {
@@ -409,6 +450,36 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   h->last.blocks = blocks;
   if (time) *time += dt;
   return (int)nfail;
+}
+
+static int eval_batch(pr_handle* h, int mode, int64_t ncells, const double* y, const double* rYsrc,
+                      const double* rhoh0, const double* rhohsrc, double t, double* out, void* stream) {
+  if (!h || !y || !rYsrc || !rhohsrc || !out || (mode == 0 && !rhoh0) || ncells <= 0)
+    return fail(PR_ERR_ARG, "bad argument");
+  HIPCHK(hipSetDevice(h->device));
+  const int G = h->var->G;
+  int block = 64;
+  for (int b = 256; b >= 64; b -= 64)
+    if ((size_t)h->blob_bytes + (size_t)(b / G) * h->var->scratch <= 65536) {
+      block = b;
+      break;
+    }
+  const size_t shmem = (size_t)h->blob_bytes + (size_t)(block / G) * h->var->scratch;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(h->var->eval(mode, h->d_blob, h->blob_bytes, h->mech->neff, h->mech->ninc, (long)ncells, y, rYsrc, rhoh0,
+                      rhohsrc, t, out, block, shmem, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int pr_rhs_batch(pr_handle* h, int64_t ncells, const double* y, const double* rYsrc, const double* rhoh0,
+                 const double* rhohsrc, double t, double* f, void* stream) {
+  return eval_batch(h, 0, ncells, y, rYsrc, rhoh0, rhohsrc, t, f, stream);
+}
+
+int pr_jac_batch(pr_handle* h, int64_t ncells, const double* y, const double* rYsrc, const double* rhohsrc,
+                 double* J, void* stream) {
+  return eval_batch(h, 1, ncells, y, rYsrc, nullptr, rhohsrc, 0.0, J, stream);
 }
 
 int pr_get_stats(const pr_handle* h, pr_stats* out) {

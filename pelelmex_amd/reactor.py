@@ -22,8 +22,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpelelmex_amd.so")
 _LIB = None
 
-EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_get_stats", "pr_num_species",
-           "pr_destroy", "pr_last_error"]
+EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_rhs_batch", "pr_jac_batch",
+           "pr_get_stats", "pr_num_species", "pr_destroy", "pr_last_error"]
 
 
 class Array4(C.Structure):
@@ -62,6 +62,10 @@ def lib():
                                    C.POINTER(Array4), C.POINTER(IArray4), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
         L.pr_react_box.restype = C.c_int
+        L.pr_rhs_batch.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_double, C.c_void_p, C.c_void_p]
+        L.pr_jac_batch.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]
         L.pr_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.pr_num_species.argtypes = [C.c_void_p]
         L.pr_destroy.argtypes = [C.c_void_p]
@@ -144,6 +148,30 @@ class Reactor:
         if rc < 0:
             raise ReactorError(f"pr_react_box failed ({rc}): {lib().pr_last_error().decode()}")
         return rc, dt.value, tm.value
+
+    def rhs_batch(self, y, rYsrc, rhoh0, rhohsrc, t=0.0, stream=None):
+        """Right-hand side for a batch: y [n, ncells], rYsrc [N, ncells] (device tensors)."""
+        import torch
+        n, nc = y.shape
+        f = torch.empty_like(y)
+        st = stream if stream is not None else torch.cuda.current_stream()
+        rc = lib().pr_rhs_batch(self._h, nc, y.data_ptr(), rYsrc.data_ptr(), rhoh0.data_ptr(), rhohsrc.data_ptr(),
+                                float(t), f.data_ptr(), C.c_void_p(st.cuda_stream))
+        if rc != 0:
+            raise ReactorError(lib().pr_last_error().decode())
+        return f
+
+    def jac_batch(self, y, rYsrc, rhohsrc, stream=None):
+        """Analytic Jacobians for a batch; returns [n, n, ncells]."""
+        import torch
+        n, nc = y.shape
+        J = torch.empty((n, n, nc), dtype=y.dtype, device=y.device)
+        st = stream if stream is not None else torch.cuda.current_stream()
+        rc = lib().pr_jac_batch(self._h, nc, y.data_ptr(), rYsrc.data_ptr(), rhohsrc.data_ptr(), J.data_ptr(),
+                                C.c_void_p(st.cuda_stream))
+        if rc != 0:
+            raise ReactorError(lib().pr_last_error().decode())
+        return J
 
     def stats(self):
         s = Stats()

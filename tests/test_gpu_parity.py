@@ -1,9 +1,11 @@
 """GPU reactor (libpelelmex_amd.so, through its C-ABI) against the CPU oracle.
 
-Tolerances are those the reactor is specified to meet against the CPU integrator at
-rtol = atol = 1e-10: max |dT|/T <= 1e-6 and max |dY_k| <= 1e-8 (Y = rhoY / rho). rho h is
-checked to round-off (it is rho h0 + dt Q_h on both sides), FC (RHS evaluations) must be
-positive on active cells and zero on masked cells.
+The GPU kernels and the oracle evaluate the same IEEE-754 expression trees (own exp / log on
+both sides, no contraction, sums in component order), so the GPU results are required to be
+bit-identical to the oracle's: right-hand sides, Jacobians, and whole react calls (rhoY, T,
+rho h, FC). The specified tolerance of the reactor against the CPU integrator at
+rtol = atol = 1e-10 (max |dT|/T <= 1e-6, max |dY_k| <= 1e-8 with Y = rhoY / rho) is checked
+as well, and is what a non-identical build would have to meet.
 """
 import numpy as np
 import pytest
@@ -50,6 +52,20 @@ def _compare(a, b, mask):
     return float(dT[act].max()), float(dY[:, act].max())
 
 
+def _ulps(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    d = np.abs(a - b)
+    sc = np.spacing(np.maximum(np.abs(a), np.abs(b)))
+    return float(np.max(np.where(d == 0, 0.0, d / np.where(sc > 0, sc, 1e-300))))
+
+
+def _assert_bits(name, got, ref):
+    if not np.array_equal(got, ref):
+        raise AssertionError(f"{name}: not bit-identical to the oracle, max {_ulps(got, ref):.3g} ulp, "
+                             f"{int(np.sum(got != ref))} of {got.size} entries differ")
+
+
 def _sample(box, n, seed=0):
     rng = np.random.default_rng(seed)
     idx = np.sort(rng.choice(box["T"].size, n, replace=False))
@@ -68,7 +84,10 @@ def test_h2_air_parity(gpu, lanes):
     dT, dY = _compare(out, ref, box["mask"])
     print(f"LiDryer G={lanes}: max dT/T {dT:.2e} max dY {dY:.2e} stats {out['stats']}")
     assert dT <= T_TOL and dY <= Y_TOL
-    np.testing.assert_allclose(out["rH"], box["rH"] + box["dt"] * box["rHsrc"], rtol=1e-15, atol=0)
+    _assert_bits("rY", out["rY"], ref["rY"])
+    _assert_bits("T", out["T"], ref["T"])
+    _assert_bits("rH", out["rH"], ref["rH"])
+    _assert_bits("FC", out["FC"], ref["FC"])
     assert np.all(out["FC"] > 0)
 
 
@@ -84,7 +103,10 @@ def test_ch4_flame_parity(gpu, lanes):
     dT, dY = _compare(out, ref, box["mask"])
     print(f"drm19 G={lanes}: max dT/T {dT:.2e} max dY {dY:.2e} stats {out['stats']}")
     assert dT <= T_TOL and dY <= Y_TOL
-    np.testing.assert_allclose(out["rH"], box["rH"] + box["dt"] * box["rHsrc"], rtol=1e-15, atol=0)
+    _assert_bits("rY", out["rY"], ref["rY"])
+    _assert_bits("T", out["T"], ref["T"])
+    _assert_bits("rH", out["rH"], ref["rH"])
+    _assert_bits("FC", out["FC"], ref["FC"])
 
 
 def test_mask_skips_cells(gpu):
@@ -102,3 +124,34 @@ def test_mask_skips_cells(gpu):
     ref = _oracle(box, max_order=5)
     dT, dY = _compare(out, ref, mask)
     assert dT <= T_TOL and dY <= Y_TOL
+    _assert_bits("rY", out["rY"], ref["rY"])
+    _assert_bits("T", out["T"], ref["T"])
+
+
+@pytest.mark.parametrize("mech_box", ["h2", "ch4"])
+def test_rhs_and_jacobian_bits(gpu, mech_box):
+    """Batched right-hand sides and Jacobians on 256 states of each workload, bit for bit."""
+    import torch
+    from oracle import pyoracle as po
+    from pelelmex_amd.reactor import Reactor
+    full = W.h2_air_box() if mech_box == "h2" else W.ch4_air_flame_box()
+    box = _sample(full, 256, seed=5)
+    m = po.Mech(box["mech"])
+    n = m.nsp + 1
+    nc = box["T"].size
+    y = np.vstack([box["rY"], box["T"][None, :]])
+    t_eval = 0.37 * box["dt"]
+    fref = np.zeros((n, nc))
+    Jref = np.zeros((n, n, nc))
+    for c in range(nc):
+        cell = m.cell(box["rH"][c], box["rHsrc"][c], box["rYsrc"][:, c])
+        fref[:, c] = m.rhs(cell, t_eval, y[:, c])
+        Jref[:, :, c] = m.jac(cell, 0.0, y[:, c])
+    r = Reactor.create("ReactorCvode", box["mech"], KEYS)
+    r.init(2, 1)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64).to(gpu)
+    f = r.rhs_batch(dev(y), dev(box["rYsrc"]), dev(box["rH"]), dev(box["rHsrc"]), t_eval).cpu().numpy()
+    J = r.jac_batch(dev(y), dev(box["rYsrc"]), dev(box["rHsrc"])).cpu().numpy()
+    r.close()
+    _assert_bits("rhs", f, fref)
+    _assert_bits("jac", J, Jref)
