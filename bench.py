@@ -1,0 +1,140 @@
+"""Benchmark: chem cells/s per dt_react of the plain GPU BDF reactor on one MI355X.
+
+Workload (one "step" = one ReactorBase::react call over the whole box, inputs resident in HBM):
+drm19 CH4/air flame-like box, 64^3 = 262144 cells of uneven stiffness (30 % cold unburnt at
+300 K, 20 % flame front with radicals, 50 % burnt products), Gaussian external sources at 5 %
+of |rhoY|/dt and |rhoH|/dt, dt_react = 5e-6 s, 1 atm, rtol = atol = 1e-10, cvode.max_order = 4
+(the value the reference's production inputs use), FP64 throughout. The box is restored from a
+pristine device copy before each react call, so every step integrates the same states.
+
+Prints one JSON line. `roofline` is the integration kernel's algorithmic FP64 rate
+(pelelmex_amd.roofline counts, kernel time from HIP events on the launch stream) against the
+MI355X FP64 peak; `cpu_baseline` is the CPU oracle (OpenMP over cells, all host threads) on a
+seeded random sample of the same box.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "chem cells/s per dt_react: surrogate+fallback vs GPU BDF at max|ΔT|/T≤1e-4"
+
+
+def cpu_baseline(box, sample, seed=7):
+    """CPU oracle on `sample` random cells of the box; returns the cpu_baseline object."""
+    from oracle import pyoracle as po
+    from pelelmex_amd import workloads as W
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(box["T"].size, sample, replace=False))
+    sub = W.subsample(box, idx)
+    m = po.Mech(box["mech"])
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    out = m.react(sub["rY"], sub["rYsrc"], sub["T"], sub["rH"], sub["rHsrc"], sub["mask"], sub["dt"],
+                  max_order=4, nthreads=nthreads)
+    dt = time.perf_counter() - t0
+    return {"value": sample / dt, "unit": "cells/s", "cores": nthreads, "kind": "port",
+            "sample": f"{sample} seeded-random cells of the same 64^3 box, CPU oracle BDF (C, OpenMP "
+                      f"dynamic over cells), {dt:.1f} s, failures {int(np.sum(out['status'] != 0))}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--lanes", type=int, default=32, help="hipbdf.lanes_per_cell")
+    ap.add_argument("--max-order", type=int, default=4)
+    ap.add_argument("--cpu-sample", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nx", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X; this build has no multi-GPU path (each GPU would run its own "
+              "boxes independently).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pelelmex_amd import roofline, workloads
+    from pelelmex_amd.reactor import Reactor, box_tensors
+
+    box = workloads.ch4_air_flame_box(nx=args.nx, ny=args.nx, nz=args.nx)
+    ncells = box["T"].size
+    dev = torch.device("cuda:0")
+    t = box_tensors(box, dev)
+    pristine = {k: t[k].clone() for k in ("rY", "T", "rH")}
+    keys = {"ode.rtol": 1e-10, "ode.atol": 1e-10, "cvode.max_order": args.max_order,
+            "cvode.max_substeps": 10000, "cvode.solve_type": "denseAJ_direct",
+            "hipbdf.lanes_per_cell": args.lanes}
+    r = Reactor.create("ReactorCvode", "drm19", keys)
+    r.init(2, 1)
+
+    def step():
+        for k in pristine:
+            t[k].copy_(pristine[k])
+        nfail, _, _ = r.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], t["mask"], box["dt"], 0.0)
+        if nfail:
+            raise RuntimeError(f"{nfail} cells failed to integrate")
+        return r.stats()
+
+    for _ in range(args.warmup):
+        step()
+    ref_T = t["T"].clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kstats = []
+    for _ in range(args.steps):
+        kstats.append(step())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    deterministic = bool(torch.equal(ref_T, t["T"])) if args.warmup > 0 else None
+    value = ncells / (ms_per_step / 1e3)
+
+    kernel_ms = float(np.mean([s["kernel_ms"] for s in kstats]))
+    st = kstats[-1]
+    F = roofline.flops("drm19", st)
+    achieved = F / (kernel_ms / 1e3) / 1e12
+    counts = roofline.static_counts("drm19")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain GPU BDF, drm19 CH4/air flame-like box {args.nx}^3 = {ncells} cells, "
+                               "dt_react 5e-6 s, 1 atm, 5% Gaussian sources",
+                   "mechanism": "drm19", "cells": ncells, "dt_react": box["dt"], "rtol": 1e-10, "atol": 1e-10,
+                   "max_order": args.max_order, "lanes_per_cell": st["lanes_per_cell"], "seed": 2002,
+                   "path": "plain BDF (surrogate not in this build)"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": roofline.MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / roofline.MI355X_FP64_TFLOPS, "traffic": None,
+                     "kernel": "react_kernel<21,84,%d>" % st["lanes_per_cell"], "kernel_ms": kernel_ms,
+                     "flops_per_launch": F,
+                     "hbm_algorithmic_bytes": counts["bytes_cell"] * ncells,
+                     "hbm_algorithmic_GBs": counts["bytes_cell"] * ncells / (kernel_ms / 1e3) / 1e9},
+        "integrator": {"steps_per_cell": st["nst"] / st["cells"], "rhs_per_cell": st["nfe"] / st["cells"],
+                       "jac_per_cell": st["nje"] / st["cells"], "lu_per_cell": st["nsetups"] / st["cells"],
+                       "newton_per_cell": st["nni"] / st["cells"], "err_fail_per_cell": st["netf"] / st["cells"],
+                       "deterministic_repeat": deterministic},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(box, min(args.cpu_sample, ncells))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
