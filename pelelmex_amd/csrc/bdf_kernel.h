@@ -14,7 +14,9 @@
 //    thermo lane-parallel over species, Arrhenius / third-body / falloff rates lane-parallel
 //    over reactions with the mechanism tables staged in LDS, production rates gathered
 //    species-parallel through per-species incidence lists (deterministic, no atomics), the
-//    Jacobian assembled row-by-row in a per-cell LDS matrix that is also the saved Jacobian.
+//    Jacobian assembled row-by-row (each lane owns its rows) in the group's global scratch
+//    slot, which also keeps it as the saved Jacobian between refreshes (L2-resident; it is
+//    touched only on Jacobian / LU refreshes, not in the Newton iterations).
 //  * the dense LU of M = I - gamma J uses partial pivoting without moving data: the pivot row
 //    of each step is broadcast across the group and each lane eliminates its own rows.
 //
@@ -61,6 +63,7 @@ struct KParams {
   unsigned long long* stats;     // cells, nst, nfe, nje, nsetups, nni, netf, ncfn
   int* status;                   // per cell (linear box index), nullable
   unsigned int* nfail;
+  double* gscr;                  // per-group global scratch slots (saved Jacobian, Jacobian records)
 };
 
 enum { ST_OK = 0, ST_TOO_MUCH_WORK = -1, ST_ERR_FAILURE = -3, ST_CONV_FAILURE = -4, ST_GUARD = -99 };
@@ -177,7 +180,7 @@ __device__ __forceinline__ void gseq(double* red, int lane, const double (&a)[RP
   }
   group_sync();
   double x = 0.0, y = 0.0;
-#pragma unroll
+#pragma unroll 4
   for (int i = 0; i < CNT; ++i) {
     x += red[i];
     if (NB > 1) y += red[CNT + i];
@@ -341,17 +344,20 @@ struct Smem {
   const double* effval;
   const int* incs;
   const int* inc;
-  double* Js;    // (N+1)^2, row-major: saved Jacobian
+  double* Js;    // global scratch slot: (N+1)^2 row-major saved Jacobian
+  double* jrec;  // global scratch slot: 4 R per-reaction Jacobian records (kf, kr, dqdM, dqdT)
   double* sC;    // N concentrations
   double* sG;    // N g/RT
   double* sH;    // N h/RT
   double* sHm;   // N mass-specific h
   double* sCpm;  // N mass-specific cp
-  double* rec;   // 5 R: kf, kr, dqdM, dqdT per reaction, then q
+  double* q;     // R rates of progress
   double* red;   // 2 (N+1): reduction buffer
   int* perm;     // N+1 pivot rows
   static constexpr int n = N + 1;
-  static constexpr int scratch_doubles = n * n + 5 * N + 5 * R + 2 * n + (n + 1) / 2 + 1;
+  // per-group LDS (doubles) and per-group global scratch slot (doubles)
+  static constexpr int scratch_doubles = 5 * N + R + 2 * n + (n + 1) / 2 + 1;
+  static constexpr int slot_doubles = n * n + 4 * R;
 };
 
 template <int N, int R, int G>
@@ -422,7 +428,7 @@ __device__ __forceinline__ void rhs_eval(const Smem<N, R>& S, int lane, double t
     if (r < R) {
       RxOut o;
       rx_eval<false>(S.rx[r], S.sC, S.sG, S.sH, Ctot, T, lnT, invT, lnRTP, S.effsp, S.effval, o);
-      S.rec[4 * R + r] = o.fwd - o.rev;
+      S.q[r] = o.fwd - o.rev;
     }
   }
   group_sync();
@@ -437,7 +443,7 @@ __device__ __forceinline__ void rhs_eval(const Smem<N, R>& S, int lane, double t
       double w = 0.0;
       for (int e = S.incs[i]; e < S.incs[i + 1]; ++e) {
         const int rn = S.inc[e];
-        w += (double)(rn >> 16) * S.rec[4 * R + (rn & 0xFFFF)];
+        w += (double)(rn >> 16) * S.q[rn & 0xFFFF];
       }
       f[s] = w * S.sp[i].W + src[s];
       pn[s] = hk[s] * f[s];
@@ -488,12 +494,12 @@ __device__ __forceinline__ void jac_eval(const Smem<N, R>& S, int lane, const do
     if (r < R) {
       RxOut o;
       rx_eval<true>(S.rx[r], S.sC, S.sG, S.sH, Ctot, T, lnT, invT, lnRTP, S.effsp, S.effval, o);
-      double* rc = S.rec + 4 * r;
+      double* rc = S.jrec + 4 * r;
       rc[0] = o.kf;
       rc[1] = o.kr;
       rc[2] = o.dqdM;
       rc[3] = o.dqdT;
-      S.rec[4 * R + r] = o.fwd - o.rev;
+      S.q[r] = o.fwd - o.rev;
     }
   }
   group_sync();
@@ -514,9 +520,9 @@ __device__ __forceinline__ void jac_eval(const Smem<N, R>& S, int lane, const do
         const int r = rn & 0xFFFF;
         const double nu = (double)(rn >> 16);
         const RxRec& rx = S.rx[r];
-        const double* rc = S.rec + 4 * r;
+        const double* rc = S.jrec + 4 * r;
         const int info = rx.info, type = info & 15, nre = (info >> 8) & 15, npr = (info >> 12) & 15;
-        w += nu * S.rec[4 * R + r];
+        w += nu * S.q[r];
         const int a0 = rx.reac & 0xFF, a1 = (rx.reac >> 8) & 0xFF, a2 = (rx.reac >> 16) & 0xFF;
         const double c0 = S.sC[a0], c1 = nre > 1 ? S.sC[a1] : 1.0, c2 = nre > 2 ? S.sC[a2] : 1.0;
         const double kf = nu * rc[0];

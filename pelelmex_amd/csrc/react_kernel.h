@@ -65,15 +65,19 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
   const int grp = threadIdx.x / G;
   double* scr = reinterpret_cast<double*>(smem + BL.bytes) + (long)grp * group_doubles<N, R>();
   Ctl& c = *reinterpret_cast<Ctl*>(scr);
-  S.Js = scr + CTL_DOUBLES;
-  S.sC = S.Js + n * n;
+  S.sC = scr + CTL_DOUBLES;
   S.sG = S.sC + N;
   S.sH = S.sG + N;
   S.sHm = S.sH + N;
   S.sCpm = S.sHm + N;
-  S.rec = S.sCpm + N;
-  S.red = S.rec + 5 * R;
+  S.q = S.sCpm + N;
+  S.red = S.q + R;
   S.perm = reinterpret_cast<int*>(S.red + 2 * n);
+  {
+    const long slot = (long)blockIdx.x * (blockDim.x / G) + grp;
+    S.Js = P.gscr + slot * Smem<N, R>::slot_doubles;
+    S.jrec = S.Js + n * n;
+  }
 
   const double tout = P.dt;
   const int qmax = P.max_order < 1 ? 1 : (P.max_order > Q ? Q : P.max_order);
@@ -744,7 +748,7 @@ template <int N, int R, int G, int MODE>
 __global__ void __launch_bounds__(BLOCK) eval_kernel(const unsigned char* blob, int blob_bytes, int neff, int ninc,
                                                      long ncells, const double* y, const double* rYsrc,
                                                      const double* rhoh0, const double* rhohsrc, double t,
-                                                     double* out) {
+                                                     double* out, double* gscr) {
   constexpr int n = N + 1;
   constexpr int RPL = Cell<N, R, G>::RPL;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -765,17 +769,19 @@ __global__ void __launch_bounds__(BLOCK) eval_kernel(const unsigned char* blob, 
   const int lane = threadIdx.x % G;
   const int grp = threadIdx.x / G;
   double* scr = reinterpret_cast<double*>(smem + BL.bytes) + (long)grp * group_doubles<N, R>();
-  S.Js = scr + CTL_DOUBLES;
-  S.sC = S.Js + n * n;
+  S.sC = scr + CTL_DOUBLES;
   S.sG = S.sC + N;
   S.sH = S.sG + N;
   S.sHm = S.sH + N;
   S.sCpm = S.sHm + N;
-  S.rec = S.sCpm + N;
-  S.red = S.rec + 5 * R;
+  S.q = S.sCpm + N;
+  S.red = S.q + R;
   S.perm = reinterpret_cast<int*>(S.red + 2 * n);
-  const long c = (long)blockIdx.x * (blockDim.x / G) + grp;
-  if (c >= ncells) return;  // whole groups leave together
+  const long slot = (long)blockIdx.x * (blockDim.x / G) + grp;
+  S.Js = gscr + slot * Smem<N, R>::slot_doubles;
+  S.jrec = S.Js + n * n;
+  const long stride = (long)gridDim.x * (blockDim.x / G);
+  for (long c = slot; c < ncells; c += stride) {  // whole groups iterate together
   double yv[RPL], src[RPL], f[RPL];
 #pragma unroll
   for (int s = 0; s < RPL; ++s) {
@@ -793,6 +799,8 @@ __global__ void __launch_bounds__(BLOCK) eval_kernel(const unsigned char* blob, 
   } else {
     jac_eval<N, R, G>(S, lane, yv, src, rhohsrc[c]);
     for (int e = lane; e < n * n; e += G) out[(long)e * ncells + c] = S.Js[e];
+    group_sync();
+  }
   }
 }
 

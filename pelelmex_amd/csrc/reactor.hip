@@ -51,7 +51,8 @@ const pr_mech_t* find_mech(const char* name) {
 using Launch = hipError_t (*)(const prk::KParams&, int blocks, int block, size_t shmem, hipStream_t);
 using EvalLaunch = hipError_t (*)(int mode, const unsigned char* blob, int blob_bytes, int neff, int ninc, long ncells,
                                   const double* y, const double* rYsrc, const double* rhoh0, const double* rhohsrc,
-                                  double t, double* out, int block, size_t shmem, hipStream_t st);
+                                  double t, double* out, double* gscr, int blocks, int block, size_t shmem,
+                                  hipStream_t st);
 using Occ = hipError_t (*)(int* nb, int block, size_t shmem);
 
 template <int N, int R, int G>
@@ -62,16 +63,14 @@ hipError_t launch_t(const prk::KParams& P, int blocks, int block, size_t shmem, 
 
 template <int N, int R, int G>
 hipError_t eval_t(int mode, const unsigned char* blob, int blob_bytes, int neff, int ninc, long ncells, const double* y,
-                  const double* rYsrc, const double* rhoh0, const double* rhohsrc, double t, double* out, int block,
-                  size_t shmem, hipStream_t st) {
-  const long groups = block / G;
-  const int blocks = (int)((ncells + groups - 1) / groups);
+                  const double* rYsrc, const double* rhoh0, const double* rhohsrc, double t, double* out, double* gscr,
+                  int blocks, int block, size_t shmem, hipStream_t st) {
   if (mode == 0)
     hipLaunchKernelGGL((prk::eval_kernel<N, R, G, 0>), dim3(blocks), dim3(block), shmem, st, blob, blob_bytes, neff,
-                       ninc, ncells, y, rYsrc, rhoh0, rhohsrc, t, out);
+                       ninc, ncells, y, rYsrc, rhoh0, rhohsrc, t, out, gscr);
   else
     hipLaunchKernelGGL((prk::eval_kernel<N, R, G, 1>), dim3(blocks), dim3(block), shmem, st, blob, blob_bytes, neff,
-                       ninc, ncells, y, rYsrc, rhoh0, rhohsrc, t, out);
+                       ninc, ncells, y, rYsrc, rhoh0, rhohsrc, t, out, gscr);
   return hipGetLastError();
 }
 
@@ -92,11 +91,13 @@ struct Variant {
   Launch launch;
   EvalLaunch eval;
   Occ occ;
-  int scratch;  // bytes per cell group
+  int scratch;  // LDS bytes per cell group
+  int slot;     // global scratch bytes per cell group
 };
 
 #define PR_VARIANT(NAME, NS, NR, G) \
-  {NAME, G, &launch_t<NS, NR, G>, &eval_t<NS, NR, G>, &occ_t<NS, NR, G>, scratch_bytes<NS, NR>()}
+  {NAME, G, &launch_t<NS, NR, G>, &eval_t<NS, NR, G>, &occ_t<NS, NR, G>, scratch_bytes<NS, NR>(), \
+   prk::Smem<NS, NR>::slot_doubles * 8}
 
 const Variant kVariants[] = {
     PR_VARIANT("LiDryer", PR_LIDRYER_NSP, PR_LIDRYER_NR, 16),
@@ -266,6 +267,8 @@ struct pr_handle {
   int blob_bytes = 0;
   double* d_atolv = nullptr;
   void* d_counters = nullptr;  // queue (4 B), nfail (4 B), stats (8 x 8 B)
+  double* d_gscr = nullptr;    // per-group global scratch slots
+  size_t gscr_bytes = 0;
   bool use_typ = false;
   pr_stats last{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -332,6 +335,26 @@ int pr_set_typ_vals(pr_handle* h, const double* typ, int n) {
 }
 
 int pr_num_species(const pr_handle* h) { return h ? h->mech->nsp : PR_ERR_ARG; }
+
+// Largest block (64..256 threads) whose LDS (tables + per-group scratch) stays within 64 KiB.
+static int pick_block(const pr_handle* h) {
+  if (h->block) return h->block;
+  const int G = h->var->G;
+  for (int b = 256; b >= 64; b -= 64)
+    if ((size_t)h->blob_bytes + (size_t)(b / G) * h->var->scratch <= 65536) return b;
+  return 64;
+}
+
+static int ensure_gscr(pr_handle* h, long slots) {
+  const size_t need = (size_t)slots * (size_t)h->var->slot;
+  if (need <= h->gscr_bytes) return 0;
+  if (h->d_gscr) (void)hipFree(h->d_gscr);
+  h->d_gscr = nullptr;
+  h->gscr_bytes = 0;
+  HIPCHK(hipMalloc(&h->d_gscr, need));
+  h->gscr_bytes = need;
+  return 0;
+}
 
 static bool arr_ok(const pr_array4* a, int ncomp) { return a && a->p && (ncomp <= 1 || a->nstride != 0); }
 
@@ -404,16 +427,7 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   P.status = status_out;
 
   const int G = h->var->G;
-  int block = h->block;
-  if (h->keys.block == 0) {
-    // largest block (64..256 threads) whose LDS (tables + per-cell scratch) stays within 64 KiB
-    block = 64;
-    for (int b = 256; b >= 64; b -= 64)
-      if ((size_t)h->blob_bytes + (size_t)(b / G) * h->var->scratch <= 65536) {
-        block = b;
-        break;
-      }
-  }
+  const int block = pick_block(h);
   const size_t shmem = (size_t)h->blob_bytes + (size_t)(block / G) * h->var->scratch;
   int dev_cus = 0, nb = 0;
   HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, h->device));
@@ -423,6 +437,11 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   const long cap = (long)nb * dev_cus;
   const long want = (groups_needed + (block / G) - 1) / (block / G);
   const int blocks = (int)(want < cap ? want : cap);
+  {
+    int rc = ensure_gscr(h, (long)blocks * (block / G));
+    if (rc) return rc;
+  }
+  P.gscr = h->d_gscr;
 
   hipLaunchKernelGGL(prk::zero_counters, dim3(1), dim3(64), 0, st, P.queue, P.stats, P.nfail);
   HIPCHK(hipGetLastError());
@@ -458,16 +477,22 @@ static int eval_batch(pr_handle* h, int mode, int64_t ncells, const double* y, c
     return fail(PR_ERR_ARG, "bad argument");
   HIPCHK(hipSetDevice(h->device));
   const int G = h->var->G;
-  int block = 64;
-  for (int b = 256; b >= 64; b -= 64)
-    if ((size_t)h->blob_bytes + (size_t)(b / G) * h->var->scratch <= 65536) {
-      block = b;
-      break;
-    }
+  const int block = pick_block(h);
   const size_t shmem = (size_t)h->blob_bytes + (size_t)(block / G) * h->var->scratch;
+  int dev_cus = 0, nb = 0;
+  HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, h->device));
+  HIPCHK(h->var->occ(&nb, block, shmem));
+  if (nb < 1) nb = 1;
+  const long want = (ncells + (block / G) - 1) / (block / G);
+  const long cap = (long)nb * dev_cus;
+  const int blocks = (int)(want < cap ? want : cap);
+  {
+    int rc = ensure_gscr(h, (long)blocks * (block / G));
+    if (rc) return rc;
+  }
   hipStream_t st = (hipStream_t)stream;
   HIPCHK(h->var->eval(mode, h->d_blob, h->blob_bytes, h->mech->neff, h->mech->ninc, (long)ncells, y, rYsrc, rhoh0,
-                      rhohsrc, t, out, block, shmem, st));
+                      rhohsrc, t, out, h->d_gscr, blocks, block, shmem, st));
   HIPCHK(hipStreamSynchronize(st));
   return 0;
 }
@@ -493,6 +518,7 @@ void pr_destroy(pr_handle* h) {
   if (h->d_blob) (void)hipFree(h->d_blob);
   if (h->d_counters) (void)hipFree(h->d_counters);
   if (h->d_atolv) (void)hipFree(h->d_atolv);
+  if (h->d_gscr) (void)hipFree(h->d_gscr);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   delete h;
