@@ -1,0 +1,290 @@
+"""Pins of the CPU oracle (CPU only): each check is chosen so that a plausible mistake in the
+oracle (a dropped term, a wrong sign or index, a transposed operand) fails at least one of them.
+
+* own exp / log: ulp distance to libm over wide ranges.
+* NASA-7 data: continuity at T_mid; cp = dh/dT (finite differences).
+* thermo vs the reference's own flame data (Exec/RegTests/*/drm19*.dat, first / last rows):
+  molecular weights reproduce the profile density, enthalpy is conserved between the unburnt
+  and burnt ends of the adiabatic flame, and every reaction of the mechanism is in detailed
+  balance at the 10 atm profile's burnt (equilibrium) end, which pins K_c (Gibbs energies).
+* rates: mass and element conservation of wdot for arbitrary states; falloff limits.
+* Jacobian: central finite differences of the frozen-T right-hand side.
+* dense LU: numpy.linalg.solve, pivoting on a zero diagonal, singular detection.
+* BDF: y' = A y against scipy.linalg.expm; Robertson against scipy Radau; reactor cells against
+  the committed Radau / VODE fixtures (tests/golden); linear invariants and an equilibrium fixed
+  point of the reactor.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy.integrate import solve_ivp
+from scipy.linalg import expm
+
+from oracle import pyoracle as po
+from pelelmex_amd import workloads as W
+from pelelmex_amd.mechanisms import PATM, RU, gen, load
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+
+
+def test_exp_log_ulps():
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.uniform(-745, 709, 20000), rng.uniform(-1, 1, 20000), [0.0, 1e-300, -1e-300]])
+    worst = 0.0
+    for x in xs:
+        ref = np.exp(x)
+        if ref < 1e-300:
+            continue
+        worst = max(worst, abs(po.exp(x) - ref) / np.spacing(ref))
+    assert worst <= 1.0
+    xs = np.concatenate([10 ** rng.uniform(-300, 300, 20000), rng.uniform(0.5, 2.0, 20000), [2.0, 1e-310, 7.0]])
+    worst = 0.0
+    for x in xs:
+        ref = np.log(x)
+        if ref == 0:
+            continue
+        worst = max(worst, abs(po.log(x) - ref) / np.spacing(abs(ref)))
+    assert worst <= 2.0
+    assert po.log(1.0) == 0.0 and po.exp(0.0) == 1.0
+
+
+@pytest.mark.parametrize("name", gen.MECHANISMS)
+def test_nasa_continuity_and_cp(name):
+    mod, rx = gen.load(name)
+    rep = gen.validate(mod, rx)
+    assert max(rep["continuity"].values()) < 5e-6
+    m = po.Mech(mod.NAME)
+    for T in (400.0, 999.0, 1500.0, 2500.0):
+        cp, h, _ = m.thermo(T)
+        d = 1e-3
+        _, hp, _ = m.thermo(T + d)
+        _, hm, _ = m.thermo(T - d)
+        dhdT = ((T + d) * hp - (T - d) * hm) / (2 * d)   # d(h/R)/dT
+        np.testing.assert_allclose(dhdT, cp, rtol=1e-7)
+
+
+def test_generated_headers_in_sync(tmp_path):
+    for name in gen.MECHANISMS:
+        committed = open(os.path.join(gen.OUT_DIR, f"{name}.h")).read()
+        out_dir = gen.OUT_DIR
+        try:
+            gen.OUT_DIR = str(tmp_path)
+            path = gen.emit(name)
+        finally:
+            gen.OUT_DIR = out_dir
+        assert open(path).read() == committed, f"{name}.h is stale; run python -m pelelmex_amd.mechanisms.gen"
+
+
+def _pmf():
+    return json.load(open(os.path.join(GOLD, "pmf_ends.json")))
+
+
+def test_pmf_species_order_and_density():
+    m = load("drm19")
+    for key, p in _pmf()["profiles"].items():
+        assert p["columns"][4:] == m.species
+        P = p["P_atm"] * PATM
+        for end in ("first", "last"):
+            r = np.array(p[end])
+            X = r[4:] / r[4:].sum()
+            rho = P * np.sum(X * m.W) / (RU * r[1])
+            assert abs(rho / r[3] - 1) < 5e-5, (key, end)
+
+
+def test_pmf_adiabatic_flame_enthalpy():
+    """Unburnt and burnt ends of each adiabatic premixed flame have the same enthalpy."""
+    m = load("drm19")
+    for key, p in _pmf()["profiles"].items():
+        h, cp = [], []
+        for end in ("first", "last"):
+            r = np.array(p[end])
+            X = r[4:] / r[4:].sum()
+            Y = X * m.W / np.sum(X * m.W)
+            h.append(m.h_mass(r[1], Y))
+            cp.append(m.cp_mass(r[1], Y))
+        dT_equiv = (h[1] - h[0]) / cp[1]
+        assert abs(dT_equiv) < 2.0, (key, dT_equiv)
+
+
+def test_pmf_burnt_end_detailed_balance():
+    """At the 10 atm profile's burnt end every reaction's forward and reverse rates agree."""
+    p = _pmf()["profiles"]["drm19_H2CH4_10Atm"]
+    r = np.array(p["last"])
+    P = p["P_atm"] * PATM
+    X = r[4:] / r[4:].sum()
+    C = X * P / (RU * r[1])
+    f, b = po.Mech("drm19").rates_fr(C, r[1])
+    big = np.maximum(f, b) > 1e-12 * np.maximum(f, b).max()
+    rel = np.abs(f - b)[big] / np.maximum(f, b)[big]
+    assert big.sum() >= 25 and rel.max() < 1e-8
+
+
+def _random_state(mech, rng, T=1500.0):
+    m = load(mech)
+    Y = rng.random(m.nsp) + 1e-3
+    Y /= Y.sum()
+    rho = m.density(PATM, T, Y)
+    return m, Y, rho
+
+
+@pytest.mark.parametrize("name", ["LiDryer", "drm19"])
+def test_mass_and_element_conservation(name):
+    rng = np.random.default_rng(3)
+    om = po.Mech(name)
+    for T in (800.0, 1500.0, 2400.0):
+        m, Y, rho = _random_state(name, rng, T)
+        C = rho * Y / m.W
+        q, w = om.rates(C, T)
+        scale = np.abs(w * m.W).max()
+        assert abs(np.sum(w * m.W)) <= 1e-13 * scale
+        el = m.elem_matrix.T @ w
+        assert np.all(np.abs(el) <= 1e-12 * np.abs(w).max())
+
+
+def test_falloff_limits():
+    """H + O2 (+M) = HO2 (+M) in LiDryer: low pressure -> k0 [M], high pressure -> k_inf F_inf."""
+    om = po.Mech("LiDryer")
+    m = load("LiDryer")
+    j = next(i for i, r in enumerate(m.reactions) if r.third == "(+M)" and "HO2" in r.prod and "H" in r.reac)
+    rxn = m.reactions[j]
+    T = 1200.0
+    R_CAL = 8.314462618 / 4.184
+    kinf = rxn.A * T ** rxn.b * np.exp(-rxn.E / (R_CAL * T))
+    k0 = rxn.low[0] * T ** rxn.low[1] * np.exp(-rxn.low[2] / (R_CAL * T))
+    Fcent = rxn.troe[0]   # (1 - a) exp(-T/1e-30) + a exp(-T/1e30) = a
+    for scale, expect in ((1e-12, "low"), (1e12, "high")):
+        X = np.zeros(m.nsp)
+        X[m.index["H"]] = 1e-3
+        X[m.index["O2"]] = 0.2
+        X[m.index["N2"]] = 1 - X.sum()
+        C = X * PATM / (RU * T) * scale
+        f, _ = om.rates_fr(C, T)
+        k = f[j] / (C[m.index["H"]] * C[m.index["O2"]])
+        M = sum(C[m.index[s]] * (rxn.eff.get(s, 1.0)) for s in m.species)
+        if expect == "low":
+            # Pr -> 0: k -> k0 [M] F, and log F -> log Fcent / (1 + (1/0.14)^2)
+            F0 = Fcent ** (1.0 / (1.0 + (1.0 / 0.14) ** 2))
+            assert abs(k / (k0 * M * F0) - 1) < 5e-3
+        else:
+            Finf = Fcent ** (1.0 / (1.0 + (1.0 / 0.14) ** 2))
+            assert abs(k / (kinf * Finf) - 1) < 5e-3
+
+
+def _cells():
+    b = W.ch4_air_flame_box(nx=64, ny=2, nz=2, seed=31)
+    h = W.h2_air_box(nx=8, ny=2, nz=2, seed=32)
+    return [(b, c) for c in (0, 25, 40, 60)] + [(h, c) for c in (0, 9)]
+
+
+@pytest.mark.parametrize("bc", range(6))
+def test_jacobian_vs_finite_differences(bc):
+    box, c = _cells()[bc]
+    m = po.Mech(box["mech"])
+    y = np.append(box["rY"][:, c], box["T"][c])
+    cell = m.cell(box["rH"][c], box["rHsrc"][c], box["rYsrc"][:, c])
+    J = m.jac(cell, 0.0, y)
+    n = y.size
+    rho = y[:-1].sum()
+    Jfd = np.zeros((n, n))
+    for j in range(n):
+        # central differences; absent species are perturbed by 1e-10 rho
+        e = 1e-4 * (max(abs(y[j]), 1e-6 * rho) if j < n - 1 else y[j])
+        yp, ym = y.copy(), y.copy()
+        yp[j] += e
+        ym[j] -= e
+        Jfd[:, j] = (m.rhs(cell, 0, yp, frozen=True) - m.rhs(cell, 0, ym, frozen=True)) / (2 * e)
+    scale = np.maximum(np.abs(J).max(axis=1, keepdims=True), 1e-300)
+    assert np.max(np.abs(J - Jfd) / scale) < 1e-6
+
+
+def test_lu_against_numpy():
+    rng = np.random.default_rng(1)
+    for n in (3, 10, 22, 54):
+        A = rng.standard_normal((n, n))
+        A[0, 0] = 0.0                       # forces a row interchange
+        b = rng.standard_normal(n)
+        LU, piv, ier = po.lu_factor(A)
+        assert ier == 0
+        x = po.lu_solve(LU, piv, b)
+        np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-9, atol=1e-12)
+    A = np.ones((4, 4))
+    _, _, ier = po.lu_factor(A)
+    assert ier > 0
+
+
+def test_bdf_linear_system_against_expm():
+    rng = np.random.default_rng(2)
+    n = 6
+    V = rng.standard_normal((n, n))
+    lam = -np.logspace(0, 4, n)
+    A = V @ np.diag(lam) @ np.linalg.inv(V)
+    y0 = rng.standard_normal(n)
+    for q in (2, 5):
+        y, ier, st = po.bdf_solve(lambda t, y: A @ y, lambda t, y: A, y0, 0.0, 1.0, 1e-10, 1e-12, max_order=q)
+        assert ier == 0
+        ref = expm(A) @ y0
+        assert np.max(np.abs(y - ref)) < 1e-7 * np.max(np.abs(ref)) + 1e-10
+
+
+def test_bdf_robertson():
+    """Robertson (1966) kinetics to t = 40 (Hairer & Wanner, Solving ODEs II, test problem ROBER)."""
+    def f(t, y):
+        return np.array([-0.04 * y[0] + 1e4 * y[1] * y[2],
+                         0.04 * y[0] - 1e4 * y[1] * y[2] - 3e7 * y[1] ** 2,
+                         3e7 * y[1] ** 2])
+
+    def jac(t, y):
+        return np.array([[-0.04, 1e4 * y[2], 1e4 * y[1]],
+                         [0.04, -1e4 * y[2] - 6e7 * y[1], -1e4 * y[1]],
+                         [0.0, 6e7 * y[1], 0.0]])
+
+    y, ier, st = po.bdf_solve(f, jac, [1.0, 0.0, 0.0], 0.0, 40.0, 1e-10, np.array([1e-14, 1e-18, 1e-14]))
+    assert ier == 0
+    ref = solve_ivp(f, (0, 40), [1.0, 0.0, 0.0], method="Radau", rtol=1e-13, atol=[1e-16, 1e-20, 1e-16],
+                    jac=jac).y[:, -1]
+    np.testing.assert_allclose(y, ref, rtol=1e-7)
+    assert abs(y.sum() - 1.0) < 1e-13          # linear invariant
+
+
+@pytest.mark.parametrize("fixture", ["h2_air_32_cells", "ch4_air_flame_64_cells"])
+def test_react_against_radau_vode_fixtures(fixture):
+    g = np.load(os.path.join(GOLD, fixture + ".npz"))
+    m = po.Mech(str(g["mech"]))
+    N = m.nsp
+    out = m.react(g["rY"], g["rYsrc"], g["T"], g["rH"], g["rHsrc"], None, float(g["dt"]), max_order=5, nthreads=4)
+    assert np.all(out["status"] == 0)
+    for ref in (g["radau"], g["vode"]):
+        dT = np.abs(out["T"] - ref[N]) / ref[N]
+        Y = out["rY"] / out["rY"].sum(0)
+        Yr = ref[:N] / ref[:N].sum(0)
+        # global error of a BDF at rtol = atol = 1e-10 over ~100 steps
+        assert dT.max() < 1e-7 and np.abs(Y - Yr).max() < 5e-8
+    assert np.abs(g["radau"] - g["vode"]).max() <= 1e-12 * np.abs(g["radau"]).max()
+
+
+def test_react_invariants_and_fixed_point():
+    box = W.subsample(W.ch4_air_flame_box(nx=64, ny=2, nz=2, seed=33), np.arange(0, 64, 4))
+    m = po.Mech("drm19")
+    lm = load("drm19")
+    out = m.react(box["rY"], box["rYsrc"], box["T"], box["rH"], box["rHsrc"], None, box["dt"], nthreads=4)
+    assert np.all(out["status"] == 0)
+    # rho h(dt) = rho h0 + dt Q_h; total mass and elements follow the sources exactly
+    np.testing.assert_array_equal(out["rH"], box["rH"] + box["dt"] * box["rHsrc"])
+    mass0 = box["rY"].sum(0) + box["dt"] * box["rYsrc"].sum(0)
+    np.testing.assert_allclose(out["rY"].sum(0), mass0, rtol=1e-12)
+    E = (lm.elem_matrix / lm.W[:, None]).T
+    np.testing.assert_allclose(E @ out["rY"], E @ (box["rY"] + box["dt"] * box["rYsrc"]), rtol=1e-11)
+    # an equilibrium state without sources stays put
+    p = _pmf()["profiles"]["drm19_H2CH4_10Atm"]
+    r = np.array(p["last"])
+    X = r[4:] / r[4:].sum()
+    Y = X * lm.W / np.sum(X * lm.W)
+    rho = 10 * PATM * np.sum(X * lm.W) / (RU * r[1])
+    rY = (rho * Y)[:, None]
+    o = m.react(rY, np.zeros_like(rY), np.array([r[1]]), np.array([rho * lm.h_mass(r[1], Y)]), np.zeros(1),
+                None, 1e-4)
+    assert o["status"][0] == 0
+    assert abs(o["T"][0] - r[1]) < 1e-3
