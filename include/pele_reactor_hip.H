@@ -1,0 +1,113 @@
+// C++ adapter with the ReactorBase member functions PeleLMeX calls, over the C-ABI of
+// pele_reactor_capi.h. Header-only; templated on the caller's Box / Array4 types so it binds to
+// AMReX's amrex::Box, amrex::Array4<Real> and amrex::Array4<int> without including AMReX.
+//
+// Reference call sites this adapter serves (PeleLMeX):
+//   m_reactor = ReactorBase::create(m_chem_integrator);   Source/PeleLMeX_Setup.cpp:152-153
+//   m_reactor->init(reactor_type = 2, ncells_chem = 1);    Source/PeleLMeX_Setup.cpp:154
+//   m_reactor->set_typ_vals_ode(typical_values);           Source/PeleLMeX_Utils.cpp:1472
+//   m_reactor->react(bx, rhoY, frcExt, T, rhoH, frcEExt, fcl, mask, dt, time, stream)
+//                                                          Source/PeleLMeX_Reactions.cpp:101-108
+//   m_reactor->close();                                    Source/PeleLMeX.cpp:26
+// A PelePhysics ReactorBase subclass registered under a factory name forwards these calls to
+// this class.
+#ifndef PELE_REACTOR_HIP_H
+#define PELE_REACTOR_HIP_H
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pele_reactor_capi.h"
+
+namespace pele_hip {
+
+class ReactorError : public std::runtime_error {
+ public:
+  explicit ReactorError(const std::string& m) : std::runtime_error(m) {}
+};
+
+// Array4 view conversion: p points at (begin.x, begin.y, begin.z, 0).
+template <class A4>
+inline pr_array4 to_pr(const A4& a) {
+  pr_array4 r;
+  r.p = const_cast<double*>(a.p);
+  r.jstride = a.jstride;
+  r.kstride = a.kstride;
+  r.nstride = a.nstride;
+  r.begin[0] = a.begin.x;
+  r.begin[1] = a.begin.y;
+  r.begin[2] = a.begin.z;
+  r.ncomp = a.ncomp;
+  return r;
+}
+
+template <class I4>
+inline pr_iarray4 to_pr_mask(const I4& a) {
+  pr_iarray4 r;
+  r.p = a.p;
+  r.jstride = a.jstride;
+  r.kstride = a.kstride;
+  r.begin[0] = a.begin.x;
+  r.begin[1] = a.begin.y;
+  r.begin[2] = a.begin.z;
+  return r;
+}
+
+class ReactorHipBDF {
+ public:
+  // integrator: the factory name PeleLMeX passes ("ReactorCvode" keeps existing inputs valid);
+  // mechanism: the compile-time Chemistry_Model of the build ("drm19", "LiDryer");
+  // keys: the ode.* / cvode.* / hipbdf.* ParmParse entries as "key = value" lines.
+  ReactorHipBDF(std::string integrator, std::string mechanism, std::string keys)
+      : integrator_(std::move(integrator)), mechanism_(std::move(mechanism)), keys_(std::move(keys)) {}
+  ReactorHipBDF(const ReactorHipBDF&) = delete;
+  ReactorHipBDF& operator=(const ReactorHipBDF&) = delete;
+  ~ReactorHipBDF() { close(); }
+
+  int init(int reactor_type, int ncells) {
+    close();
+    int rc = pr_create(integrator_.c_str(), mechanism_.c_str(), reactor_type, ncells,
+                       keys_.empty() ? nullptr : keys_.c_str(), &h_);
+    if (rc != 0) throw ReactorError(std::string("pr_create: ") + pr_last_error());
+    return 0;
+  }
+
+  void set_typ_vals_ode(const std::vector<double>& typ) {
+    int rc = pr_set_typ_vals(h_, typ.data(), static_cast<int>(typ.size()));
+    if (rc != 0) throw ReactorError(std::string("pr_set_typ_vals: ") + pr_last_error());
+  }
+
+  // Box form of react (device arrays, CGS units). Returns the number of cells that failed to
+  // integrate; like the reference, the caller may ignore it. Blocks until the stream is done.
+  template <class Box, class A4, class I4>
+  int react(const Box& box, A4 const& rY, A4 const& rYsrc, A4 const& T, A4 const& rE, A4 const& rEsrc,
+            A4 const& FC, I4 const& mask, double& dt_react, double& time, void* stream = nullptr) {
+    const int lo[3] = {box.smallEnd(0), box.smallEnd(1), box.smallEnd(2)};
+    const int hi[3] = {box.bigEnd(0), box.bigEnd(1), box.bigEnd(2)};
+    pr_array4 a[6] = {to_pr(rY), to_pr(rYsrc), to_pr(T), to_pr(rE), to_pr(rEsrc), to_pr(FC)};
+    pr_iarray4 m = to_pr_mask(mask);
+    int rc = pr_react_box(h_, lo, hi, &a[0], &a[1], &a[2], &a[3], &a[4], &a[5], &m, &dt_react, &time, stream,
+                          nullptr);
+    if (rc < 0) throw ReactorError(std::string("pr_react_box: ") + pr_last_error());
+    return rc;
+  }
+
+  void close() {
+    if (h_) pr_destroy(h_);
+    h_ = nullptr;
+  }
+
+  pr_stats stats() const {
+    pr_stats s{};
+    pr_get_stats(h_, &s);
+    return s;
+  }
+
+ private:
+  std::string integrator_, mechanism_, keys_;
+  pr_handle* h_ = nullptr;
+};
+
+}  // namespace pele_hip
+#endif

@@ -91,3 +91,33 @@ def test_workloads_deterministic_and_physical():
     assert np.all(a["T"][x >= 0.5] >= 1800)
     h = W.h2_air_box(nx=8, ny=2, nz=2)
     assert np.all((h["T"] >= 1000) & (h["T"] <= 1800)) and np.all(h["rYsrc"] == 0)
+
+
+def test_cpp_adapter_compiles_and_links(tmp_path):
+    """include/pele_reactor_hip.H against stand-ins of amrex::Box / Array4, linked to the library."""
+    if not os.path.exists(R.LIB_PATH):
+        pytest.skip("library not built")
+    src = tmp_path / "adapter.cpp"
+    src.write_text(r'''
+#include "pele_reactor_hip.H"
+#include <cstdio>
+struct Dim3 { int x, y, z; };
+struct IntVect { int v[3]; int operator[](int i) const { return v[i]; } };
+struct Box { IntVect lo, hi; int smallEnd(int d) const { return lo[d]; } int bigEnd(int d) const { return hi[d]; } };
+template <class T> struct Array4 { T* p; long long jstride, kstride, nstride; Dim3 begin, end; int ncomp; };
+int main() {
+  pele_hip::ReactorHipBDF r("ReactorCvode", "drm19", "ode.rtol = -5");
+  try { r.init(2, 1); } catch (const pele_hip::ReactorError& e) { std::printf("%s\n", e.what()); return 0; }
+  return 1;
+}
+template int pele_hip::ReactorHipBDF::react<Box, Array4<double>, Array4<int>>(
+    const Box&, Array4<double> const&, Array4<double> const&, Array4<double> const&, Array4<double> const&,
+    Array4<double> const&, Array4<double> const&, Array4<int> const&, double&, double&, void*);
+''')
+    import subprocess
+    exe = tmp_path / "adapter"
+    libdir = os.path.dirname(R.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                    "-L", libdir, "-lpelelmex_amd", f"-Wl,-rpath,{libdir}"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True)
+    assert "ode.rtol" in out.stdout
