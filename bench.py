@@ -39,7 +39,7 @@ def cpu_baseline(box, sample, seed=7):
                   max_order=4, nthreads=nthreads)
     dt = time.perf_counter() - t0
     return {"value": sample / dt, "unit": "cells/s", "cores": nthreads, "kind": "port",
-            "sample": f"{sample} seeded-random cells of the same 64^3 box, CPU oracle BDF (C, OpenMP "
+            "sample": f"{sample} cells (seeded random order) of the same 64^3 box, CPU oracle BDF (C, OpenMP "
                       f"dynamic over cells), {dt:.1f} s, failures {int(np.sum(out['status'] != 0))}"}
 
 
@@ -50,9 +50,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--lanes", type=int, default=32, help="hipbdf.lanes_per_cell")
     ap.add_argument("--max-order", type=int, default=4)
-    ap.add_argument("--cpu-sample", type=int, default=4096)
+    ap.add_argument("--cpu-sample", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nx", type=int, default=64)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run one plain device copy of this size (calibrates byte counters in a profile)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X; this build has no multi-GPU path (each GPU would run its own "
@@ -82,6 +84,13 @@ def main():
             raise RuntimeError(f"{nfail} cells failed to integrate")
         return r.stats()
 
+    if args.copy_probe_mb:
+        a = torch.empty(args.copy_probe_mb * (1 << 20) // 8, dtype=torch.float64, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     for _ in range(args.warmup):
         step()
     ref_T = t["T"].clone()
