@@ -111,12 +111,12 @@ typedef struct {
 } bdf_t;
 
 static double wrms(const bdf_t* b, const double* v) {
-  double s = 0.0;
+  double sq[ORC_NMAX];
   for (int i = 0; i < b->n; ++i) {
     double t = v[i] * b->ewt[i];
-    s += t * t;
+    sq[i] = t * t;
   }
-  return sqrt(s / b->n);
+  return sqrt(orc_psum(sq, b->n) / b->n);
 }
 
 static void set_ewt(bdf_t* b, const double* y) {

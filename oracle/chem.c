@@ -17,7 +17,8 @@
  *    (Source/PeleLMeX_Reactions.cpp:101-108, Source/PeleLMeX_EB.cpp:157-161, PeleLMeX.H:209).
  *
  * Arithmetic is written as explicit IEEE-754 expression trees (no implicit contraction, fma
- * where stated) with sums taken in component order and production rates / Jacobian rows
+ * where stated) with sums over components in one canonical pairwise order (orc_psum) and
+ * production rates / Jacobian rows
  * gathered species by species over each species' reactions in ascending order. The HIP
  * kernels evaluate the same trees, so GPU and oracle agree bit for bit; the oracle itself is
  * pinned by tests against finite differences, conservation laws and scipy integrators.
@@ -44,6 +45,16 @@ int orc_mech_nsp(int id) { return MECHS[id]->nsp; }
 int orc_mech_nr(int id) { return MECHS[id]->nr; }
 void orc_mech_weights(int id, double* W) {
   for (int k = 0; k < MECHS[id]->nsp; ++k) W[k] = MECHS[id]->W[k];
+}
+
+/* Sum of x[0..n-1] in the canonical pairwise order: zero-padded to 64 slots, then pairs
+   (2j, 2j+1), pairs of pairs, ... (the order the GPU's lane butterflies produce). */
+double orc_psum(const double* x, int n) {
+  double b[64];
+  for (int i = 0; i < 64; ++i) b[i] = (i < n) ? x[i] : 0.0;
+  for (int w = 1; w < 64; w *= 2)
+    for (int j = 0; j < 64; j += 2 * w) b[j] = b[j] + b[j + w];
+  return b[0];
 }
 
 /* Derived per-species tables: R/W, 1/W and the NASA-7 polynomials rewritten for Horner
@@ -264,7 +275,7 @@ static void state_at(int id, const double* y, double T, state_t* s) {
     s->cpk[k] = cpR * p->RW;
     s->dcpk[k] = dcpR * p->RW;
   }
-  for (int k = 0; k < m->nsp; ++k) s->Ctot += s->C[k];
+  s->Ctot = orc_psum(s->C, m->nsp);
 }
 
 /* production rates by the species-wise gather over each species' reactions */
@@ -283,11 +294,8 @@ void orc_rates(int id, const double* C, double T, double* q, double* wdot) {
   state_t s;
   state_at(id, y, T, &s);
   /* use the given concentrations exactly */
-  s.Ctot = 0.0;
-  for (int k = 0; k < m->nsp; ++k) {
-    s.C[k] = C[k];
-    s.Ctot += C[k];
-  }
+  for (int k = 0; k < m->nsp; ++k) s.C[k] = C[k];
+  s.Ctot = orc_psum(s.C, m->nsp);
   for (int r = 0; r < m->nr; ++r) {
     rx_info o;
     reaction(m, r, s.C, s.Ctot, T, s.lnT, s.invT, s.lnRTP, s.gRT, s.hRT, 0, &o);
@@ -303,11 +311,8 @@ void orc_rates_fr(int id, const double* C, double T, double* fwd, double* rev) {
   for (int k = 0; k < m->nsp; ++k) y[k] = C[k] * m->W[k];
   state_t s;
   state_at(id, y, T, &s);
-  s.Ctot = 0.0;
-  for (int k = 0; k < m->nsp; ++k) {
-    s.C[k] = C[k];
-    s.Ctot += C[k];
-  }
+  for (int k = 0; k < m->nsp; ++k) s.C[k] = C[k];
+  s.Ctot = orc_psum(s.C, m->nsp);
   for (int r = 0; r < m->nr; ++r) {
     rx_info o;
     reaction(m, r, s.C, s.Ctot, T, s.lnT, s.invT, s.lnRTP, s.gRT, s.hRT, 0, &o);
@@ -328,15 +333,16 @@ static int hy2t_rhoY(int id, double h, const double* rY, double rho, double* T) 
   double t = *T;
   int it;
   for (it = 0; it < HY2T_MAXIT; ++it) {
-    double invT = 1.0 / t, hm = 0.0, cpm = 0.0;
+    double invT = 1.0 / t, ph[ORC_NMAX], pc[ORC_NMAX];
     for (int k = 0; k < m->nsp; ++k) {
       const sp_t* p = &SP[id][k];
       double cpR, hRT;
       nasa_hcp(p, t, invT, &cpR, &hRT);
       double Y = rY[k] / rho;
-      hm += ((Y * hRT) * p->RW) * t;
-      cpm += (Y * cpR) * p->RW;
+      ph[k] = ((Y * hRT) * p->RW) * t;
+      pc[k] = (Y * cpR) * p->RW;
     }
+    double hm = orc_psum(ph, m->nsp), cpm = orc_psum(pc, m->nsp);
     double dT = (h - hm) / cpm;
     t = t + dT;
     if (fabs(dT) <= HY2T_RTOL * t) {
@@ -364,17 +370,19 @@ static void rhs_at_T(int id, const orc_cell* c, const double* y, double T, doubl
   }
   gather_wdot(m, q, wdot);
   for (int k = 0; k < N; ++k) f[k] = wdot[k] * m->W[k] + c->rYsrc[k];
-  double pn = 0.0, rcp = 0.0;
-  for (int k = 0; k < N; ++k) pn += s.hk[k] * f[k];
-  for (int k = 0; k < N; ++k) rcp += y[k] * s.cpk[k];
+  double a[ORC_NMAX], b[ORC_NMAX];
+  for (int k = 0; k < N; ++k) {
+    a[k] = s.hk[k] * f[k];
+    b[k] = y[k] * s.cpk[k];
+  }
+  double pn = orc_psum(a, N), rcp = orc_psum(b, N);
   f[N] = (c->rhohsrc - pn) / rcp;
 }
 
 int orc_reactor_rhs(double t, const double* y, double* f, void* cell) {
   const orc_cell* c = (const orc_cell*)cell;
   const int N = MECHS[c->mech]->nsp;
-  double rho = 0.0;
-  for (int k = 0; k < N; ++k) rho += y[k];
+  double rho = orc_psum(y, N);
   double h = (c->rhoh0 + t * c->rhohsrc) / rho;
   double T = y[N];
   hy2t_rhoY(c->mech, h, y, rho, &T);
@@ -438,11 +446,14 @@ int orc_reactor_jac(double t, const double* y, const double* fy, double* J, void
     row[N] = JT * Wk;
     fk[k] = w * Wk + c->rYsrc[k];
   }
-  double pn = 0.0, rcp = 0.0, sdcp = 0.0, scf = 0.0;
-  for (int k = 0; k < N; ++k) pn += s.hk[k] * fk[k];
-  for (int k = 0; k < N; ++k) rcp += y[k] * s.cpk[k];
-  for (int k = 0; k < N; ++k) sdcp += y[k] * s.dcpk[k];
-  for (int k = 0; k < N; ++k) scf += s.cpk[k] * fk[k];
+  double a1[ORC_NMAX], a2[ORC_NMAX], a3[ORC_NMAX], a4[ORC_NMAX];
+  for (int k = 0; k < N; ++k) {
+    a1[k] = s.hk[k] * fk[k];
+    a2[k] = y[k] * s.cpk[k];
+    a3[k] = y[k] * s.dcpk[k];
+    a4[k] = s.cpk[k] * fk[k];
+  }
+  double pn = orc_psum(a1, N), rcp = orc_psum(a2, N), sdcp = orc_psum(a3, N), scf = orc_psum(a4, N);
   const double fT = (c->rhohsrc - pn) / rcp;
   for (int j = 0; j < n; ++j) {
     double cs = 0.0;

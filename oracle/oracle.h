@@ -60,6 +60,9 @@ int orc_bdf_solve(int n, double t0, double tout, double* y, orc_rhs_fn f, orc_ja
 int orc_lu_factor(int n, double* A, int* piv);
 void orc_lu_solve(int n, const double* LU, const int* piv, double* b);
 
+/* Sum in the canonical pairwise order (zero-padded to 64 slots, pairs (2j, 2j+1) first). */
+double orc_psum(const double* x, int n);
+
 /* ---------------- mechanism kernels ---------------- */
 int orc_mech_count(void);
 const pr_mech_t* orc_mech_get(int id);

@@ -22,9 +22,9 @@
 //
 // Arithmetic: every expression is an explicit IEEE-754 tree (the library is built without
 // floating-point contraction; fma only where written), exp / log are this file's own
-// routines, and every sum over components is taken sequentially in component order through
-// an LDS buffer. Results therefore do not depend on G or on the launch shape, and they are
-// bit-identical to the CPU oracle, which evaluates the same trees.
+// routines, and every sum over components follows one canonical pairwise tree (gpsum).
+// Results therefore do not depend on G or on the launch shape, and they are bit-identical to
+// the CPU oracle, which evaluates the same trees.
 #ifndef PR_BDF_KERNEL_H
 #define PR_BDF_KERNEL_H
 #include <hip/hip_runtime.h>
@@ -165,42 +165,37 @@ __device__ __forceinline__ double gmax(double v) {
   return v;
 }
 
-// Sums over components 0..CNT-1 of per-row values (one or two series), in component order,
-// returned to every lane of the group. `red` is the group's LDS buffer (2 x (N+1) doubles).
-template <int G, int RPL, int CNT, int NB>
-__device__ __forceinline__ void gseq(double* red, int lane, const double (&a)[RPL], const double (&b)[RPL],
-                                     double& sa, double& sb) {
+// Sum over components 0..CNT-1 of per-row values, returned to every lane of the group, in the
+// canonical pairwise order: components zero-padded to 64 slots, then summed in pairs (2j, 2j+1),
+// pairs of pairs, ... . An ascending-xor butterfly over the G lanes forms exactly that tree
+// within each block of G components (lane l of block s holds component l + G s); the block sums
+// are then combined pairwise up to 64 slots. The result is independent of G and equals the
+// oracle's pairwise sum bit for bit.
+template <int G, int RPL, int CNT>
+__device__ __forceinline__ double gpsum(int lane, const double (&a)[RPL]) {
+  constexpr int NB = 64 / G;
+  static_assert(RPL <= NB, "at most 64 components");
+  double t[NB];
 #pragma unroll
-  for (int s = 0; s < RPL; ++s) {
-    const int i = lane + G * s;
-    if (i < CNT) {
-      red[i] = a[s];
-      if (NB > 1) red[CNT + i] = b[s];
-    }
+  for (int s = 0; s < NB; ++s) {
+    double v = 0.0;
+    if (s < RPL) v = (lane + G * s < CNT) ? a[s < RPL ? s : 0] : 0.0;
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) v += __shfl_xor(v, m, G);
+    t[s] = v;
   }
-  group_sync();
-  double x = 0.0, y = 0.0;
-#pragma unroll 4
-  for (int i = 0; i < CNT; ++i) {
-    x += red[i];
-    if (NB > 1) y += red[CNT + i];
-  }
-  sa = x;
-  sb = y;
-  group_sync();
+#pragma unroll
+  for (int w = 1; w < NB; w <<= 1)
+#pragma unroll
+    for (int j = 0; j < NB; j += 2 * w) t[j] = t[j] + t[j + w];
+  return t[0];
 }
 
 template <int G, int RPL, int CNT>
-__device__ __forceinline__ double gseq1(double* red, int lane, const double (&a)[RPL]) {
-  double x, y;
-  gseq<G, RPL, CNT, 1>(red, lane, a, a, x, y);
-  return x;
-}
-
-template <int G, int RPL, int CNT>
-__device__ __forceinline__ void gseq2(double* red, int lane, const double (&a)[RPL], const double (&b)[RPL],
-                                      double& sa, double& sb) {
-  gseq<G, RPL, CNT, 2>(red, lane, a, b, sa, sb);
+__device__ __forceinline__ void gpsum2(int lane, const double (&a)[RPL], const double (&b)[RPL], double& sa,
+                                       double& sb) {
+  sa = gpsum<G, RPL, CNT>(lane, a);
+  sb = gpsum<G, RPL, CNT>(lane, b);
 }
 
 // ---------------------------------------------------------------- thermo and rates
@@ -352,11 +347,10 @@ struct Smem {
   double* sHm;   // N mass-specific h
   double* sCpm;  // N mass-specific cp
   double* q;     // R rates of progress
-  double* red;   // 2 (N+1): reduction buffer
   int* perm;     // N+1 pivot rows
   static constexpr int n = N + 1;
   // per-group LDS (doubles) and per-group global scratch slot (doubles)
-  static constexpr int scratch_doubles = 5 * N + R + 2 * n + (n + 1) / 2 + 1;
+  static constexpr int scratch_doubles = 5 * N + R + (n + 1) / 2 + 1;
   static constexpr int slot_doubles = n * n + 4 * R;
 };
 
@@ -375,7 +369,7 @@ __device__ __forceinline__ void rhs_eval(const Smem<N, R>& S, int lane, double t
                                          const double (&src)[Cell<N, R, G>::RPL], double rhoh0,
                                          double rhohsrc, double (&f)[Cell<N, R, G>::RPL]) {
   constexpr int RPL = Cell<N, R, G>::RPL;
-  const double rho = gseq1<G, RPL, N>(S.red, lane, y);
+  const double rho = gpsum<G, RPL, N>(lane, y);
   const double hmix = (rhoh0 + t * rhohsrc) / rho;
   double T = gbcast<G>(y[N / G], N % G);
   for (int it = 0; it < 20; ++it) {
@@ -396,7 +390,7 @@ __device__ __forceinline__ void rhs_eval(const Smem<N, R>& S, int lane, double t
       }
     }
     double hm, cpm;
-    gseq2<G, RPL, N>(S.red, lane, ph, pc, hm, cpm);
+    gpsum2<G, RPL, N>(lane, ph, pc, hm, cpm);
     const double dT = (hmix - hm) / cpm;
     T = T + dT;
     if (fabs(dT) <= 1e-10 * T) break;
@@ -421,7 +415,7 @@ __device__ __forceinline__ void rhs_eval(const Smem<N, R>& S, int lane, double t
       cpk[s] = cpR * sp.RW;
     }
   }
-  const double Ctot = gseq1<G, RPL, N>(S.red, lane, cc);
+  const double Ctot = gpsum<G, RPL, N>(lane, cc);
 #pragma unroll 1
   for (int m = 0; m < Cell<N, R, G>::RXL; ++m) {
     const int r = lane + G * m;
@@ -451,7 +445,7 @@ __device__ __forceinline__ void rhs_eval(const Smem<N, R>& S, int lane, double t
     }
   }
   double sn, rcp;
-  gseq2<G, RPL, N>(S.red, lane, pn, pcp, sn, rcp);
+  gpsum2<G, RPL, N>(lane, pn, pcp, sn, rcp);
   const double fT = (rhohsrc - sn) / rcp;
   if (lane == N % G) f[N / G] = fT;
 }
@@ -487,7 +481,7 @@ __device__ __forceinline__ void jac_eval(const Smem<N, R>& S, int lane, const do
       dcpk[s] = dcp * sp.RW;
     }
   }
-  const double Ctot = gseq1<G, RPL, N>(S.red, lane, cc);
+  const double Ctot = gpsum<G, RPL, N>(lane, cc);
 #pragma unroll 1
   for (int m = 0; m < Cell<N, R, G>::RXL; ++m) {
     const int r = lane + G * m;
@@ -554,8 +548,8 @@ __device__ __forceinline__ void jac_eval(const Smem<N, R>& S, int lane, const do
     }
   }
   double sn, rcp, sdcp, scf;
-  gseq2<G, RPL, N>(S.red, lane, pn, prcp, sn, rcp);
-  gseq2<G, RPL, N>(S.red, lane, psd, pcf, sdcp, scf);
+  gpsum2<G, RPL, N>(lane, pn, prcp, sn, rcp);
+  gpsum2<G, RPL, N>(lane, psd, pcf, sdcp, scf);
   const double fT = (rhohsrc - sn) / rcp;
 #pragma unroll
   for (int s = 0; s < RPL; ++s) {
@@ -687,7 +681,7 @@ __device__ __forceinline__ double wrms(const Smem<N, R>& S, int lane, const doub
     const double t = v[s] * w[s];
     sq[s] = t * t;
   }
-  const double ss = gseq1<G, RPL, N + 1>(S.red, lane, sq);
+  const double ss = gpsum<G, RPL, N + 1>(lane, sq);
   return sqrt(ss / (N + 1));
 }
 

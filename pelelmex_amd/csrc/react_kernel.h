@@ -71,8 +71,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
   S.sHm = S.sH + N;
   S.sCpm = S.sHm + N;
   S.q = S.sCpm + N;
-  S.red = S.q + R;
-  S.perm = reinterpret_cast<int*>(S.red + 2 * n);
+  S.perm = reinterpret_cast<int*>(S.q + R);
   {
     const long slot = (long)blockIdx.x * (blockDim.x / G) + grp;
     S.Js = P.gscr + slot * Smem<N, R>::slot_doubles;
@@ -775,8 +774,7 @@ __global__ void __launch_bounds__(BLOCK) eval_kernel(const unsigned char* blob, 
   S.sHm = S.sH + N;
   S.sCpm = S.sHm + N;
   S.q = S.sCpm + N;
-  S.red = S.q + R;
-  S.perm = reinterpret_cast<int*>(S.red + 2 * n);
+  S.perm = reinterpret_cast<int*>(S.q + R);
   const long slot = (long)blockIdx.x * (blockDim.x / G) + grp;
   S.Js = gscr + slot * Smem<N, R>::slot_doubles;
   S.jrec = S.Js + n * n;

@@ -144,9 +144,20 @@ std::vector<unsigned char> pack_blob(const pr_mech_t* m) {
     o.RW = PR_RU / m->W[k];
     o.tmid = m->nasa[15 * k];
   }
+  // Reactions are stored grouped by kind (Troe, Lindemann, third body, elementary; stable within
+  // a kind) so that the lanes of one rate round mostly take the same code path. Incidence lists
+  // keep the mechanism's reaction order and point at the stored slots, so production rates and
+  // Jacobian rows are still summed in mechanism order.
+  std::vector<int> slot_of(m->nr);
+  {
+    int next = 0;
+    for (int kind = PR_RX_TROE; kind >= PR_RX_ELEM; --kind)
+      for (int r = 0; r < m->nr; ++r)
+        if (m->rtype[r] == kind) slot_of[r] = next++;
+  }
   auto* rx = reinterpret_cast<prk::RxRec*>(b.data() + L.off_rx);
   for (int r = 0; r < m->nr; ++r) {
-    prk::RxRec& o = rx[r];
+    prk::RxRec& o = rx[slot_of[r]];
     o.lnA = m->arr[3 * r];
     o.beta = m->arr[3 * r + 1];
     o.Ta = m->arr[3 * r + 2];
@@ -173,7 +184,7 @@ std::vector<unsigned char> pack_blob(const pr_mech_t* m) {
   auto* incs = reinterpret_cast<int*>(b.data() + L.off_incs);
   auto* inc = reinterpret_cast<int*>(b.data() + L.off_inc);
   for (int k = 0; k <= m->nsp; ++k) incs[k] = m->inc_start[k];
-  for (int e = 0; e < m->ninc; ++e) inc[e] = (m->inc_r[e] & 0xFFFF) | (m->inc_nu[e] * 65536);
+  for (int e = 0; e < m->ninc; ++e) inc[e] = (slot_of[m->inc_r[e]] & 0xFFFF) | (m->inc_nu[e] * 65536);
   return b;
 }
 
