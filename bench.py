@@ -23,6 +23,42 @@ import time
 import numpy as np
 
 METRIC = "chem cells/s per dt_react: surrogate+fallback vs GPU BDF at max|ΔT|/T≤1e-4"
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PROFILE_DIR = os.path.join(ROOT, "profiles")
+
+
+def kernel_source_digest():
+    """sha256 over the device sources the react kernel is built from (ties a profile to a build)."""
+    import hashlib
+    csrc = os.path.join(ROOT, "pelelmex_amd", "csrc")
+    h = hashlib.sha256()
+    for dirpath, _, files in sorted(os.walk(csrc)):
+        for f in sorted(files):
+            if f.endswith((".h", ".hip")):
+                p = os.path.join(dirpath, f)
+                h.update(os.path.relpath(p, csrc).encode())
+                h.update(open(p, "rb").read())
+    return h.hexdigest()[:16]
+
+
+def profiled_traffic(workload_key):
+    """HBM bytes per launch of the react kernel from a committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (profiles/*.md) of this workload and these exact kernel sources; None otherwise."""
+    if not os.path.isdir(PROFILE_DIR):
+        return None, None
+    digest = kernel_source_digest()
+    for f in sorted(os.listdir(PROFILE_DIR)):
+        if not f.endswith(".md"):
+            continue
+        kv = {}
+        for line in open(os.path.join(PROFILE_DIR, f)):
+            if ": " in line and not line.startswith(("|", "#", "`")):
+                k, v = line.rstrip("\n").split(": ", 1)
+                kv[k.strip()] = v.strip()
+        if kv.get("workload_key") == workload_key and kv.get("source_digest") == digest and \
+                "traffic_bytes_per_launch" in kv:
+            return float(kv["traffic_bytes_per_launch"]), f
+    return None, None
 
 
 def cpu_baseline(box, sample, seed=7):
@@ -110,6 +146,8 @@ def main():
     F = roofline.flops("drm19", st)
     achieved = F / (kernel_ms / 1e3) / 1e12
     counts = roofline.static_counts("drm19")
+    workload_key = f"drm19 {args.nx}^3 lanes {st['lanes_per_cell']} max_order {args.max_order}"
+    traffic, traffic_src = profiled_traffic(workload_key)
 
     line = {
         "metric": METRIC,
@@ -130,7 +168,9 @@ def main():
                    "max_order": args.max_order, "lanes_per_cell": st["lanes_per_cell"], "seed": 2002,
                    "path": "plain BDF (surrogate not in this build)"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": roofline.MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / roofline.MI355X_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / roofline.MI355X_FP64_TFLOPS, "traffic": traffic,
+                     "traffic_source": (f"profiles/{traffic_src} (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a "
+                                        "1 GiB device copy)") if traffic else "no profile of these kernel sources",
                      "kernel": "react_kernel<21,84,%d>" % st["lanes_per_cell"], "kernel_ms": kernel_ms,
                      "flops_per_launch": F,
                      "hbm_algorithmic_bytes": counts["bytes_cell"] * ncells,
