@@ -141,9 +141,35 @@ __device__ __forceinline__ void group_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// One cell per wavefront (G == 64): cross-lane moves stay on the VALU. DPP permutes within rows
+// of 16 lanes (quad_perm [1,0,3,2] = lane ^ 1, quad_perm [2,3,0,1] = lane ^ 2, row_half_mirror,
+// row_mirror); readlane moves a lane's value into scalar registers, so reductions and
+// broadcasts return wave-uniform values. Every caller runs with all 64 lanes active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
+// value of lane l (l wave-uniform)
+__device__ __forceinline__ double rlane_d(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+// wave-uniform copy of a value that is identical in all lanes
+template <int G>
+__device__ __forceinline__ int uni(int x) {
+  if constexpr (G == 64) return __builtin_amdgcn_readfirstlane(x);
+  else return x;
+}
+
 template <int G>
 __device__ __forceinline__ double gbcast(double v, int src) {
-  return __shfl(v, src, G);
+  if constexpr (G == 64) return rlane_d(v, __builtin_amdgcn_readfirstlane(src));
+  else return __shfl(v, src, G);
 }
 
 template <int G>
@@ -160,9 +186,17 @@ __device__ __forceinline__ void gargmax(double& v, int& idx) {
 
 template <int G>
 __device__ __forceinline__ double gmax(double v) {
+  if constexpr (G == 64) {
+    v = fmax(v, dpp_d<0xB1>(v));
+    v = fmax(v, dpp_d<0x4E>(v));
+    v = fmax(v, dpp_d<0x141>(v));  // quad-uniform values: the mirrored quad is lane ^ 4
+    v = fmax(v, dpp_d<0x140>(v));  // 8-uniform values: the mirrored half row is lane ^ 8
+    return fmax(fmax(rlane_d(v, 0), rlane_d(v, 16)), fmax(rlane_d(v, 32), rlane_d(v, 48)));
+  } else {
 #pragma unroll
-  for (int m = G / 2; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, G));
-  return v;
+    for (int m = G / 2; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, G));
+    return v;
+  }
 }
 
 // Sum over components 0..CNT-1 of per-row values, returned to every lane of the group, in the
@@ -175,6 +209,15 @@ template <int G, int RPL, int CNT>
 __device__ __forceinline__ double gpsum(int lane, const double (&a)[RPL]) {
   constexpr int NB = 64 / G;
   static_assert(RPL <= NB, "at most 64 components");
+  if constexpr (G == 64) {
+    // the same tree: pairs, quads, octets and rows of 16 by DPP, then (row0 + row1) + (row2 + row3)
+    double v = (lane < CNT) ? a[0] : 0.0;
+    v = v + dpp_d<0xB1>(v);
+    v = v + dpp_d<0x4E>(v);
+    v = v + dpp_d<0x141>(v);
+    v = v + dpp_d<0x140>(v);
+    return (rlane_d(v, 0) + rlane_d(v, 16)) + (rlane_d(v, 32) + rlane_d(v, 48));
+  }
   double t[NB];
 #pragma unroll
   for (int s = 0; s < NB; ++s) {
@@ -359,6 +402,11 @@ struct Cell {
   static constexpr int n = N + 1;
   static constexpr int RPL = (n + G - 1) / G;
   static constexpr int RXL = (R + G - 1) / G;
+  // With one cell per wavefront the saved Jacobian and its records sit in LDS after the group's
+  // scratch (few groups per block); narrower groups keep them in a global scratch slot.
+  static constexpr bool js_lds = (G == 64);
+  static constexpr int lds_doubles = Smem<N, R>::scratch_doubles + (js_lds ? Smem<N, R>::slot_doubles : 0);
+  static constexpr int gslot_doubles = js_lds ? 0 : Smem<N, R>::slot_doubles;
 };
 
 // ---------------------------------------------------------------- right-hand side
@@ -434,8 +482,19 @@ __device__ __forceinline__ void rhs_eval(const Smem<N, R>& S, int lane, double t
     pn[s] = 0.0;
     pcp[s] = 0.0;
     if (i < N) {
+      // production rate, summed in incidence order; the table reads are issued four at a time
       double w = 0.0;
-      for (int e = S.incs[i]; e < S.incs[i + 1]; ++e) {
+      const int e1 = S.incs[i + 1];
+      int e = S.incs[i];
+      for (; e + 4 <= e1; e += 4) {
+        const int a0 = S.inc[e], a1 = S.inc[e + 1], a2 = S.inc[e + 2], a3 = S.inc[e + 3];
+        const double q0 = S.q[a0 & 0xFFFF], q1 = S.q[a1 & 0xFFFF], q2 = S.q[a2 & 0xFFFF], q3 = S.q[a3 & 0xFFFF];
+        w += (double)(a0 >> 16) * q0;
+        w += (double)(a1 >> 16) * q1;
+        w += (double)(a2 >> 16) * q2;
+        w += (double)(a3 >> 16) * q3;
+      }
+      for (; e < e1; ++e) {
         const int rn = S.inc[e];
         w += (double)(rn >> 16) * S.q[rn & 0xFFFF];
       }
@@ -565,14 +624,50 @@ __device__ __forceinline__ void jac_eval(const Smem<N, R>& S, int lane, const do
 
 // ---------------------------------------------------------------- dense LU and solve
 // LU of M = I - gamma * Js with partial pivoting (first maximal |pivot| in row order).
-// Returns true when a zero pivot is met. Rows stay in the owning lanes.
+// Returns true when a zero pivot is met. Rows stay in the owning lanes; pst[s] is the step at
+// which row lane + G s became pivot row, rinv the reciprocal of its pivot, and the row pivoted
+// at step k is kept in S.perm[k] (G < 64) or in lane k of permv (G == 64).
 template <int N, int R, int G>
 __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double gamma,
                                           double (&lu)[Cell<N, R, G>::RPL][N + 1],
-                                          int (&pst)[Cell<N, R, G>::RPL], double (&rinv)[Cell<N, R, G>::RPL]) {
+                                          int (&pst)[Cell<N, R, G>::RPL], double (&rinv)[Cell<N, R, G>::RPL],
+                                          int& permv) {
   constexpr int RPL = Cell<N, R, G>::RPL;
   constexpr int n = N + 1;
   constexpr int UNPIV = 1 << 20;
+  if constexpr (G == 64) {
+    // The pivot owner is wave-uniform: the pivot row is read with readlane, no LDS round trips.
+    pst[0] = (lane < n) ? UNPIV : -1;
+    rinv[0] = 0.0;
+#pragma unroll
+    for (int j = 0; j < n; ++j) lu[0][j] = (lane < n) ? ((lane == j ? 1.0 : 0.0) - gamma * S.Js[lane * n + j]) : 0.0;
+    bool singular = false;
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+      double cand = -1.0;
+      if (pst[0] == UNPIV) {
+        const double v = fabs(lu[0][k]);
+        cand = (v == v) ? v : -2.0;
+      }
+      const double best = gmax<64>(cand);
+      if (!(best > 0.0)) {
+        singular = true;
+        break;
+      }
+      const int owner = (int)__builtin_ctzll(__ballot(cand == best));  // first maximal row
+      if (lane == owner) pst[0] = k;
+      if (lane == k) permv = owner;
+      const double inv = 1.0 / rlane_d(lu[0][k], owner);
+      if (lane == owner) rinv[0] = inv;
+      if (pst[0] > k) lu[0][k] *= inv;
+#pragma unroll
+      for (int j = k + 1; j < n; ++j) {
+        const double pj = rlane_d(lu[0][j], owner);
+        if (pst[0] > k) lu[0][j] = fma(-lu[0][k], pj, lu[0][j]);
+      }
+    }
+    return singular;
+  }
 #pragma unroll
   for (int s = 0; s < RPL; ++s) {
     const int i = lane + G * s;
@@ -638,9 +733,26 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
 template <int N, int R, int G>
 __device__ __forceinline__ void lu_solve(const Smem<N, R>& S, int lane, const double (&lu)[Cell<N, R, G>::RPL][N + 1],
                                          const int (&pst)[Cell<N, R, G>::RPL], const double (&rinv)[Cell<N, R, G>::RPL],
-                                         double (&b)[Cell<N, R, G>::RPL], double (&x)[Cell<N, R, G>::RPL]) {
+                                         int permv, double (&b)[Cell<N, R, G>::RPL], double (&x)[Cell<N, R, G>::RPL]) {
   constexpr int RPL = Cell<N, R, G>::RPL;
   constexpr int n = N + 1;
+  if constexpr (G == 64) {
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+      const int p = __builtin_amdgcn_readlane(permv, k);
+      const double yk = rlane_d(b[0], p);
+      if (pst[0] > k) b[0] = fma(-lu[0][k], yk, b[0]);
+    }
+    x[0] = 0.0;
+#pragma unroll
+    for (int k = n - 1; k >= 0; --k) {
+      const int p = __builtin_amdgcn_readlane(permv, k);
+      const double xk = rlane_d(b[0], p) * rlane_d(rinv[0], p);
+      if (pst[0] >= 0 && pst[0] < k) b[0] = fma(-lu[0][k], xk, b[0]);
+      if (lane == k) x[0] = xk;
+    }
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < n; ++k) {
     const int p = S.perm[k];

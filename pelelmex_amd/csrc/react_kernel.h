@@ -36,9 +36,39 @@ __device__ __forceinline__ long aoff(const Arr& a, int i, int j, int k) {
 
 __device__ __forceinline__ void clobber() { asm volatile("" ::: "memory"); }
 
-template <int N, int R>
+template <int N, int R, int G>
 constexpr int group_doubles() {
-  return Smem<N, R>::scratch_doubles + CTL_DOUBLES;
+  return Cell<N, R, G>::lds_doubles + CTL_DOUBLES;
+}
+
+// Views of the mechanism tables and of the group's scratch (LDS control block first, then the
+// per-group arrays, then - for one cell per wavefront - the saved Jacobian and its records).
+template <int N, int R, int G>
+__device__ __forceinline__ double* setup_views(Smem<N, R>& S, unsigned char* smem, const BlobLayout& BL, int grp,
+                                               double* gscr) {
+  constexpr int n = N + 1;
+  S.sp = reinterpret_cast<const SpRec*>(smem + BL.off_sp);
+  S.rx = reinterpret_cast<const RxRec*>(smem + BL.off_rx);
+  S.effsp = reinterpret_cast<const int*>(smem + BL.off_effsp);
+  S.effval = reinterpret_cast<const double*>(smem + BL.off_effval);
+  S.incs = reinterpret_cast<const int*>(smem + BL.off_incs);
+  S.inc = reinterpret_cast<const int*>(smem + BL.off_inc);
+  double* scr = reinterpret_cast<double*>(smem + BL.bytes) + (long)grp * group_doubles<N, R, G>();
+  S.sC = scr + CTL_DOUBLES;
+  S.sG = S.sC + N;
+  S.sH = S.sG + N;
+  S.sHm = S.sH + N;
+  S.sCpm = S.sHm + N;
+  S.q = S.sCpm + N;
+  S.perm = reinterpret_cast<int*>(S.q + R);
+  if constexpr (Cell<N, R, G>::js_lds) {
+    S.Js = S.q + R + (n + 1) / 2 + 1;
+  } else {
+    const long slot = (long)blockIdx.x * (blockDim.x / G) + grp;
+    S.Js = gscr + slot * Cell<N, R, G>::gslot_doubles;
+  }
+  S.jrec = S.Js + n * n;
+  return scr;
 }
 
 template <int N, int R, int G>
@@ -55,28 +85,9 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
   __syncthreads();
   const BlobLayout BL = blob_layout(N, R, P.neff, P.ninc);
   Smem<N, R> S;
-  S.sp = reinterpret_cast<const SpRec*>(smem + BL.off_sp);
-  S.rx = reinterpret_cast<const RxRec*>(smem + BL.off_rx);
-  S.effsp = reinterpret_cast<const int*>(smem + BL.off_effsp);
-  S.effval = reinterpret_cast<const double*>(smem + BL.off_effval);
-  S.incs = reinterpret_cast<const int*>(smem + BL.off_incs);
-  S.inc = reinterpret_cast<const int*>(smem + BL.off_inc);
   const int lane = threadIdx.x % G;
   const int grp = threadIdx.x / G;
-  double* scr = reinterpret_cast<double*>(smem + BL.bytes) + (long)grp * group_doubles<N, R>();
-  Ctl& c = *reinterpret_cast<Ctl*>(scr);
-  S.sC = scr + CTL_DOUBLES;
-  S.sG = S.sC + N;
-  S.sH = S.sG + N;
-  S.sHm = S.sH + N;
-  S.sCpm = S.sHm + N;
-  S.q = S.sCpm + N;
-  S.perm = reinterpret_cast<int*>(S.q + R);
-  {
-    const long slot = (long)blockIdx.x * (blockDim.x / G) + grp;
-    S.Js = P.gscr + slot * Smem<N, R>::slot_doubles;
-    S.jrec = S.Js + n * n;
-  }
+  Ctl& c = *reinterpret_cast<Ctl*>(setup_views<N, R, G>(S, smem, BL, grp, P.gscr));
 
   const double tout = P.dt;
   const int qmax = P.max_order < 1 ? 1 : (P.max_order > Q ? Q : P.max_order);
@@ -89,6 +100,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
   double lu[RPL][n];
   int pst[RPL];
   double rinv[RPL];
+  int permv = 0;
 #pragma unroll
   for (int s = 0; s < RPL; ++s) {
     int i = lane + G * s;
@@ -352,7 +364,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
 
   for (;;) {
     clobber();
-    if (c.phase == PH_FETCH) {
+    if (uni<G>(c.phase) == PH_FETCH) {
       unsigned int cl = 0;
       if (lane == 0) cl = atomicAdd(P.queue, 1u);
       cl = __shfl(cl, 0, G);
@@ -409,7 +421,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
     rhs_eval<N, R, G>(S, lane, c.t_eval, yv, src, c.rhoh0, c.rhohsrc, f);
     clobber();
     c.nfe += 1;
-    const int phase = c.phase;
+    const int phase = uni<G>(c.phase);
 
     if (phase == PH_F0) {
       double hinv = 0.0;
@@ -485,7 +497,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
         for (int s = 0; s < RPL; ++s) res[s] = rl1 * zn[1][s] + acor[s] - gamma * f[s];
       }
       bool singular = false;
-      if (c.callSetup) {
+      if (uni<G>(c.callSetup)) {
         const int cf = c.jbad ? CF_BADJ : c.convfail;
         const double dgamma = fabs(c.gamma / c.gammap - 1.0);
         const bool jnew = (c.nst == 0) || (c.nst >= c.nstlj + 51) || (cf == CF_BADJ && dgamma < 0.2) || (cf == CF_OTHER);
@@ -498,7 +510,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
         } else {
           c.jcur = 0;
         }
-        singular = lu_factor<N, R, G>(S, lane, c.gamma, lu, pst, rinv);
+        singular = lu_factor<N, R, G>(S, lane, c.gamma, lu, pst, rinv, permv);
         clobber();
         c.nset += 1;
         c.gamrat = 1.0;
@@ -513,7 +525,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
         double b[RPL], dlt[RPL];
 #pragma unroll
         for (int s = 0; s < RPL; ++s) b[s] = -res[s];
-        lu_solve<N, R, G>(S, lane, lu, pst, rinv, b, dlt);
+        lu_solve<N, R, G>(S, lane, lu, pst, rinv, permv, b, dlt);
         clobber();
         const double gamrat = c.gamrat;
         if (gamrat != 1.0) {
@@ -759,25 +771,10 @@ __global__ void __launch_bounds__(BLOCK) eval_kernel(const unsigned char* blob, 
   __syncthreads();
   const BlobLayout BL = blob_layout(N, R, neff, ninc);
   Smem<N, R> S;
-  S.sp = reinterpret_cast<const SpRec*>(smem + BL.off_sp);
-  S.rx = reinterpret_cast<const RxRec*>(smem + BL.off_rx);
-  S.effsp = reinterpret_cast<const int*>(smem + BL.off_effsp);
-  S.effval = reinterpret_cast<const double*>(smem + BL.off_effval);
-  S.incs = reinterpret_cast<const int*>(smem + BL.off_incs);
-  S.inc = reinterpret_cast<const int*>(smem + BL.off_inc);
   const int lane = threadIdx.x % G;
   const int grp = threadIdx.x / G;
-  double* scr = reinterpret_cast<double*>(smem + BL.bytes) + (long)grp * group_doubles<N, R>();
-  S.sC = scr + CTL_DOUBLES;
-  S.sG = S.sC + N;
-  S.sH = S.sG + N;
-  S.sHm = S.sH + N;
-  S.sCpm = S.sHm + N;
-  S.q = S.sCpm + N;
-  S.perm = reinterpret_cast<int*>(S.q + R);
+  setup_views<N, R, G>(S, smem, BL, grp, gscr);
   const long slot = (long)blockIdx.x * (blockDim.x / G) + grp;
-  S.Js = gscr + slot * Smem<N, R>::slot_doubles;
-  S.jrec = S.Js + n * n;
   const long stride = (long)gridDim.x * (blockDim.x / G);
   for (long c = slot; c < ncells; c += stride) {  // whole groups iterate together
   double yv[RPL], src[RPL], f[RPL];

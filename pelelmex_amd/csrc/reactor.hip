@@ -80,9 +80,9 @@ hipError_t occ_t(int* nb, int block, size_t shmem) {
       nb, reinterpret_cast<const void*>(&prk::react_kernel<N, R, G>), block, shmem);
 }
 
-template <int N, int R>
+template <int N, int R, int G>
 constexpr int scratch_bytes() {
-  return prk::group_doubles<N, R>() * 8;
+  return prk::group_doubles<N, R, G>() * 8;
 }
 
 struct Variant {
@@ -96,8 +96,8 @@ struct Variant {
 };
 
 #define PR_VARIANT(NAME, NS, NR, G) \
-  {NAME, G, &launch_t<NS, NR, G>, &eval_t<NS, NR, G>, &occ_t<NS, NR, G>, scratch_bytes<NS, NR>(), \
-   prk::Smem<NS, NR>::slot_doubles * 8}
+  {NAME, G, &launch_t<NS, NR, G>, &eval_t<NS, NR, G>, &occ_t<NS, NR, G>, scratch_bytes<NS, NR, G>(), \
+   prk::Cell<NS, NR, G>::gslot_doubles * 8}
 
 const Variant kVariants[] = {
     PR_VARIANT("LiDryer", PR_LIDRYER_NSP, PR_LIDRYER_NR, 16),

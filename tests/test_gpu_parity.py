@@ -128,9 +128,10 @@ def test_mask_skips_cells(gpu):
     _assert_bits("T", out["T"], ref["T"])
 
 
-@pytest.mark.parametrize("mech_box", ["h2", "ch4"])
-def test_rhs_and_jacobian_bits(gpu, mech_box):
-    """Batched right-hand sides and Jacobians on 256 states of each workload, bit for bit."""
+@pytest.mark.parametrize("mech_box,lanes", [("h2", 16), ("h2", 8), ("ch4", 32), ("ch4", 16), ("ch4", 64)])
+def test_rhs_and_jacobian_bits(gpu, mech_box, lanes):
+    """Batched right-hand sides and Jacobians on 256 states of each workload, bit for bit, for
+    every lanes-per-cell variant (cross-lane sums by shuffles, or DPP / readlane at 64)."""
     import torch
     from oracle import pyoracle as po
     from pelelmex_amd.reactor import Reactor
@@ -147,7 +148,7 @@ def test_rhs_and_jacobian_bits(gpu, mech_box):
         cell = m.cell(box["rH"][c], box["rHsrc"][c], box["rYsrc"][:, c])
         fref[:, c] = m.rhs(cell, t_eval, y[:, c])
         Jref[:, :, c] = m.jac(cell, 0.0, y[:, c])
-    r = Reactor.create("ReactorCvode", box["mech"], KEYS)
+    r = Reactor.create("ReactorCvode", box["mech"], dict(KEYS, **{"hipbdf.lanes_per_cell": lanes}))
     r.init(2, 1)
     dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64).to(gpu)
     f = r.rhs_batch(dev(y), dev(box["rYsrc"]), dev(box["rH"]), dev(box["rHsrc"]), t_eval).cpu().numpy()
