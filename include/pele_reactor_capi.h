@@ -22,6 +22,9 @@
  *                    for a batch of states, one evaluation per cell
  *   pr_jac_batch     the analytic Jacobian the integrator factors (PelePhysics' aJacobian path,
  *                    PELE_COMPILE_AJACOBIAN in Exec/Make.PeleLMeX:100-104), one per cell
+ *   pr_reaction_rate_box  reactionRateRhoY over a box (Source/PeleLMeX_K.H:848-889), as
+ *                    PeleLM::computeInstantaneousReactionRate runs it per box
+ *                    (Source/PeleLMeX_Reactions.cpp:399-435)
  */
 #ifndef PELE_REACTOR_CAPI_H
 #define PELE_REACTOR_CAPI_H
@@ -84,6 +87,13 @@ int pr_rhs_batch(pr_handle* h, int64_t ncells, const double* y, const double* rY
                  const double* rhohsrc, double t, double* f, void* stream);
 int pr_jac_batch(pr_handle* h, int64_t ncells, const double* y, const double* rYsrc, const double* rhohsrc,
                  double* J, void* stream);
+/* Instantaneous production rates rhoYdot = W_k wdot_k of every cell of [lo, hi], MKS in and out
+   as in the reference (rY kg m^-3, rH J m^-3, T K; rYdot kg m^-3 s^-1): T is recomputed from
+   (rH / rho, Y) starting at the stored T (T itself is not written). Cells with mask == -1 get
+   zero rates (mask may be NULL). Enqueues on `stream` and waits for it. */
+int pr_reaction_rate_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY,
+                         const pr_array4* rH, const pr_array4* T, const pr_array4* rYdot,
+                         const pr_iarray4* mask, void* stream);
 int pr_get_stats(const pr_handle* h, pr_stats* out);
 int pr_num_species(const pr_handle* h);
 void pr_destroy(pr_handle* h);

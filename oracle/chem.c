@@ -356,6 +356,37 @@ static int hy2t_rhoY(int id, double h, const double* rY, double rho, double* T) 
 
 int orc_hy2t(int id, double h, const double* Y, double* T) { return hy2t_rhoY(id, h, Y, 1.0, T); }
 
+/* reactionRateRhoY (Source/PeleLMeX_K.H:848-889), MKS in and out, for ncells cells stored
+   component-major (rY[k * ncells + c]). Steps as the reference: rho = sum rhoY, Y = rhoY / rho,
+   h = rhoH / rho * 1e4, T by HY2T from the stored T, then rhoYdot_k = W_k wdot_k(rho * 1e-3, T, Y)
+   * 1000. Tout (nullable) receives the temperatures. */
+void orc_rate_rhoY(int id, long ncells, const double* rY, const double* rH, const double* T0, double* out,
+                   double* Tout) {
+  const pr_mech_t* m = MECHS[id];
+  const int N = m->nsp;
+  for (long c = 0; c < ncells; ++c) {
+    double ry[ORC_NMAX], Y[ORC_NMAX], yc[ORC_NMAX], q[1024], wdot[ORC_NMAX];
+    for (int k = 0; k < N; ++k) ry[k] = rY[(long)k * ncells + c];
+    const double rho = orc_psum(ry, N);
+    const double rhoinv = 1.0 / rho;
+    for (int k = 0; k < N; ++k) Y[k] = ry[k] * rhoinv;
+    double T = T0[c];
+    hy2t_rhoY(id, (rH[c] * rhoinv) * 1.0e4, Y, 1.0, &T);
+    const double rho_cgs = rho * 0.001;
+    for (int k = 0; k < N; ++k) yc[k] = rho_cgs * Y[k];
+    state_t s;
+    state_at(id, yc, T, &s);
+    for (int r = 0; r < m->nr; ++r) {
+      rx_info o;
+      reaction(m, r, s.C, s.Ctot, T, s.lnT, s.invT, s.lnRTP, s.gRT, s.hRT, 0, &o);
+      q[r] = o.fwd - o.rev;
+    }
+    gather_wdot(m, q, wdot);
+    for (int k = 0; k < N; ++k) out[(long)k * ncells + c] = (wdot[k] * m->W[k]) * 1000.0;
+    if (Tout) Tout[c] = T;
+  }
+}
+
 /* Shared tail of both right-hand sides, evaluated at temperature T. */
 static void rhs_at_T(int id, const orc_cell* c, const double* y, double T, double* f) {
   const pr_mech_t* m = MECHS[id];

@@ -82,6 +82,9 @@ double orc_exp(double x);
 double orc_log(double x);
 /* T from mass-specific enthalpy h (erg/g) and Y by Newton from Tguess; returns iterations */
 int orc_hy2t(int id, double h, const double* Y, double* T);
+/* reactionRateRhoY on ncells cells, MKS, component-major arrays; Tout nullable */
+void orc_rate_rhoY(int id, long ncells, const double* rY, const double* rH, const double* T0, double* out,
+                   double* Tout);
 
 /* Reactor right-hand side on y = (rhoY_1..N, T) (CGS) with rho h(t) = rhoh0 + t*rhohsrc. */
 typedef struct {

@@ -62,6 +62,7 @@ def lib():
                                 C.POINTER(BdfOpts), C.c_int, ip, C.POINTER(C.c_long)]
         L.orc_mech_weights.argtypes = [C.c_int, dp]
         L.orc_rates_fr.argtypes = [C.c_int, dp, C.c_double, dp, dp]
+        L.orc_rate_rhoY.argtypes = [C.c_int, C.c_long, dp, dp, dp, dp, dp]
         L.orc_exp.argtypes = [C.c_double]
         L.orc_exp.restype = C.c_double
         L.orc_log.argtypes = [C.c_double]
@@ -108,6 +109,17 @@ class Mech:
         f, r = np.zeros(self.nr), np.zeros(self.nr)
         lib().orc_rates_fr(self.id, _p(Cmol), float(T), _p(f), _p(r))
         return f, r
+
+    def rate_rhoY(self, rY, rH, T0):
+        """reactionRateRhoY on cells stored component-major (rY shape (N, ncells)); MKS.
+        Returns (rhoYdot (N, ncells), T (ncells))."""
+        rY = np.ascontiguousarray(rY, dtype=np.float64)
+        rH = np.ascontiguousarray(rH, dtype=np.float64)
+        T0 = np.ascontiguousarray(T0, dtype=np.float64)
+        nc = rH.size
+        out, T = np.zeros((self.nsp, nc)), np.zeros(nc)
+        lib().orc_rate_rhoY(self.id, nc, _p(rY), _p(rH), _p(T0), _p(out), _p(T))
+        return out, T
 
     def hy2t(self, h, Y, Tguess):
         Y = np.ascontiguousarray(Y, dtype=np.float64)

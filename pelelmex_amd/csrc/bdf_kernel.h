@@ -410,16 +410,13 @@ struct Cell {
 };
 
 // ---------------------------------------------------------------- right-hand side
-// Reactor right-hand side f(t, y) on the group; T from (h(t), Y) by Newton, started at y_T.
+// T at fixed mass-specific enthalpy h and mass fractions Y (rows of the group), by Newton from
+// the initial value T: dT = (h - sum Y_k h_k(T)) / sum Y_k cp_k(T) until |dT| <= 1e-10 T
+// (at most 20 iterations). Eos HY2T as used at Source/PeleLMeX_K.H:879 and inside the reactor RHS.
 template <int N, int R, int G>
-__device__ __forceinline__ void rhs_eval(const Smem<N, R>& S, int lane, double t,
-                                         const double (&y)[Cell<N, R, G>::RPL],
-                                         const double (&src)[Cell<N, R, G>::RPL], double rhoh0,
-                                         double rhohsrc, double (&f)[Cell<N, R, G>::RPL]) {
+__device__ __forceinline__ double hy2t(const Smem<N, R>& S, int lane, const double (&Y)[Cell<N, R, G>::RPL],
+                                       double h, double T) {
   constexpr int RPL = Cell<N, R, G>::RPL;
-  const double rho = gpsum<G, RPL, N>(lane, y);
-  const double hmix = (rhoh0 + t * rhohsrc) / rho;
-  double T = gbcast<G>(y[N / G], N % G);
   for (int it = 0; it < 20; ++it) {
     const double invT = 1.0 / T;
     double ph[RPL], pc[RPL];
@@ -432,38 +429,45 @@ __device__ __forceinline__ void rhs_eval(const Smem<N, R>& S, int lane, double t
         const SpRec& sp = S.sp[i];
         double cpR, hRT;
         nasa_hcp(sp, T, invT, cpR, hRT);
-        const double Y = y[s] / rho;
-        ph[s] = ((Y * hRT) * sp.RW) * T;
-        pc[s] = (Y * cpR) * sp.RW;
+        ph[s] = ((Y[s] * hRT) * sp.RW) * T;
+        pc[s] = (Y[s] * cpR) * sp.RW;
       }
     }
     double hm, cpm;
     gpsum2<G, RPL, N>(lane, ph, pc, hm, cpm);
-    const double dT = (hmix - hm) / cpm;
+    const double dT = (h - hm) / cpm;
     T = T + dT;
     if (fabs(dT) <= 1e-10 * T) break;
   }
+  return T;
+}
+
+// Net molar production rates w_k (mol cm^-3 s^-1) at temperature T and concentrations cc
+// (mol cm^-3, rows of the group) - productionRate of the mechanism - plus the mass-specific
+// enthalpies hk (erg g^-1) and heat capacities cpk (erg g^-1 K^-1) at T.
+template <int N, int R, int G>
+__device__ __forceinline__ void net_rates(const Smem<N, R>& S, int lane, double T,
+                                          const double (&cc)[Cell<N, R, G>::RPL], double (&w)[Cell<N, R, G>::RPL],
+                                          double (&hk)[Cell<N, R, G>::RPL], double (&cpk)[Cell<N, R, G>::RPL]) {
+  constexpr int RPL = Cell<N, R, G>::RPL;
   const double lnT = pr_log(T), invT = 1.0 / T, lnRTP = pr_log((PR_RU * T) / PR_PATM);
-  double hk[RPL], cpk[RPL], cc[RPL];
 #pragma unroll
   for (int s = 0; s < RPL; ++s) {
     const int i = lane + G * s;
     hk[s] = 0.0;
     cpk[s] = 0.0;
-    cc[s] = 0.0;
     if (i < N) {
       const SpRec& sp = S.sp[i];
       double cpR, hRT, gRT, dcp;
       nasa_full(sp, T, invT, lnT, cpR, hRT, gRT, dcp);
-      const double C = y[s] * sp.invW;
-      S.sC[i] = C;
+      S.sC[i] = cc[s];
       S.sG[i] = gRT;
-      cc[s] = C;
       hk[s] = (hRT * sp.RW) * T;
       cpk[s] = cpR * sp.RW;
     }
   }
   const double Ctot = gpsum<G, RPL, N>(lane, cc);
+  group_sync();
 #pragma unroll 1
   for (int m = 0; m < Cell<N, R, G>::RXL; ++m) {
     const int r = lane + G * m;
@@ -474,6 +478,53 @@ __device__ __forceinline__ void rhs_eval(const Smem<N, R>& S, int lane, double t
     }
   }
   group_sync();
+#pragma unroll
+  for (int s = 0; s < RPL; ++s) {
+    const int i = lane + G * s;
+    w[s] = 0.0;
+    if (i < N) {
+      // summed in incidence order; the table reads are issued four at a time
+      double a = 0.0;
+      const int e1 = S.incs[i + 1];
+      int e = S.incs[i];
+      for (; e + 4 <= e1; e += 4) {
+        const int a0 = S.inc[e], a1 = S.inc[e + 1], a2 = S.inc[e + 2], a3 = S.inc[e + 3];
+        const double q0 = S.q[a0 & 0xFFFF], q1 = S.q[a1 & 0xFFFF], q2 = S.q[a2 & 0xFFFF], q3 = S.q[a3 & 0xFFFF];
+        a += (double)(a0 >> 16) * q0;
+        a += (double)(a1 >> 16) * q1;
+        a += (double)(a2 >> 16) * q2;
+        a += (double)(a3 >> 16) * q3;
+      }
+      for (; e < e1; ++e) {
+        const int rn = S.inc[e];
+        a += (double)(rn >> 16) * S.q[rn & 0xFFFF];
+      }
+      w[s] = a;
+    }
+  }
+  group_sync();
+}
+
+// Reactor right-hand side f(t, y) on the group; T from (h(t), Y) by Newton, started at y_T.
+// y = (rho Y_1 .. rho Y_N, T) in CGS; f_k = W_k w_k + src_k, f_T = (src_h - sum h_k f_k) / (sum rho Y_k cp_k).
+template <int N, int R, int G>
+__device__ __forceinline__ void rhs_eval(const Smem<N, R>& S, int lane, double t,
+                                         const double (&y)[Cell<N, R, G>::RPL],
+                                         const double (&src)[Cell<N, R, G>::RPL], double rhoh0,
+                                         double rhohsrc, double (&f)[Cell<N, R, G>::RPL]) {
+  constexpr int RPL = Cell<N, R, G>::RPL;
+  const double rho = gpsum<G, RPL, N>(lane, y);
+  const double hmix = (rhoh0 + t * rhohsrc) / rho;
+  double Y[RPL], cc[RPL];
+#pragma unroll
+  for (int s = 0; s < RPL; ++s) {
+    const int i = lane + G * s;
+    Y[s] = (i < N) ? y[s] / rho : 0.0;
+    cc[s] = (i < N) ? y[s] * S.sp[i].invW : 0.0;
+  }
+  const double T = hy2t<N, R, G>(S, lane, Y, hmix, gbcast<G>(y[N / G], N % G));
+  double w[RPL], hk[RPL], cpk[RPL];
+  net_rates<N, R, G>(S, lane, T, cc, w, hk, cpk);
   double pn[RPL], pcp[RPL];
 #pragma unroll
   for (int s = 0; s < RPL; ++s) {
@@ -482,23 +533,7 @@ __device__ __forceinline__ void rhs_eval(const Smem<N, R>& S, int lane, double t
     pn[s] = 0.0;
     pcp[s] = 0.0;
     if (i < N) {
-      // production rate, summed in incidence order; the table reads are issued four at a time
-      double w = 0.0;
-      const int e1 = S.incs[i + 1];
-      int e = S.incs[i];
-      for (; e + 4 <= e1; e += 4) {
-        const int a0 = S.inc[e], a1 = S.inc[e + 1], a2 = S.inc[e + 2], a3 = S.inc[e + 3];
-        const double q0 = S.q[a0 & 0xFFFF], q1 = S.q[a1 & 0xFFFF], q2 = S.q[a2 & 0xFFFF], q3 = S.q[a3 & 0xFFFF];
-        w += (double)(a0 >> 16) * q0;
-        w += (double)(a1 >> 16) * q1;
-        w += (double)(a2 >> 16) * q2;
-        w += (double)(a3 >> 16) * q3;
-      }
-      for (; e < e1; ++e) {
-        const int rn = S.inc[e];
-        w += (double)(rn >> 16) * S.q[rn & 0xFFFF];
-      }
-      f[s] = w * S.sp[i].W + src[s];
+      f[s] = w[s] * S.sp[i].W + src[s];
       pn[s] = hk[s] * f[s];
       pcp[s] = y[s] * cpk[s];
     }
@@ -507,6 +542,34 @@ __device__ __forceinline__ void rhs_eval(const Smem<N, R>& S, int lane, double t
   gpsum2<G, RPL, N>(lane, pn, pcp, sn, rcp);
   const double fT = (rhohsrc - sn) / rcp;
   if (lane == N % G) f[N / G] = fT;
+}
+
+// reactionRateRhoY (Source/PeleLMeX_K.H:848-889) on one cell, MKS in and out: rho = sum rhoY,
+// Y = rhoY / rho, h = rhoH / rho * 1e4 (erg / g), T by hy2t from the stored T, then
+// rhoYdot_k = W_k w_k(rho * 1e-3, T, Y) * 1000.
+template <int N, int R, int G>
+__device__ __forceinline__ void rate_rhoY_eval(const Smem<N, R>& S, int lane, const double (&rY)[Cell<N, R, G>::RPL],
+                                               double rH, double T0, double (&out)[Cell<N, R, G>::RPL]) {
+  constexpr int RPL = Cell<N, R, G>::RPL;
+  const double rho = gpsum<G, RPL, N>(lane, rY);
+  const double rhoinv = 1.0 / rho;
+  double Y[RPL], cc[RPL];
+#pragma unroll
+  for (int s = 0; s < RPL; ++s) Y[s] = (lane + G * s < N) ? rY[s] * rhoinv : 0.0;
+  const double T = hy2t<N, R, G>(S, lane, Y, (rH * rhoinv) * 1.0e4, T0);
+  const double rho_cgs = rho * 0.001;
+#pragma unroll
+  for (int s = 0; s < RPL; ++s) {
+    const int i = lane + G * s;
+    cc[s] = (i < N) ? (rho_cgs * Y[s]) * S.sp[i].invW : 0.0;
+  }
+  double w[RPL], hk[RPL], cpk[RPL];
+  net_rates<N, R, G>(S, lane, T, cc, w, hk, cpk);
+#pragma unroll
+  for (int s = 0; s < RPL; ++s) {
+    const int i = lane + G * s;
+    out[s] = (i < N) ? (w[s] * S.sp[i].W) * 1000.0 : 0.0;
+  }
 }
 
 // ---------------------------------------------------------------- analytic Jacobian

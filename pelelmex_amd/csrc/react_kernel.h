@@ -63,11 +63,15 @@ __device__ __forceinline__ double* setup_views(Smem<N, R>& S, unsigned char* sme
   S.perm = reinterpret_cast<int*>(S.q + R);
   if constexpr (Cell<N, R, G>::js_lds) {
     S.Js = S.q + R + (n + 1) / 2 + 1;
-  } else {
+    S.jrec = S.Js + n * n;
+  } else if (gscr) {
     const long slot = (long)blockIdx.x * (blockDim.x / G) + grp;
     S.Js = gscr + slot * Cell<N, R, G>::gslot_doubles;
+    S.jrec = S.Js + n * n;
+  } else {
+    S.Js = nullptr;  // kernels without a Jacobian
+    S.jrec = nullptr;
   }
-  S.jrec = S.Js + n * n;
   return scr;
 }
 
@@ -796,6 +800,68 @@ __global__ void __launch_bounds__(BLOCK) eval_kernel(const unsigned char* blob, 
     for (int e = lane; e < n * n; e += G) out[(long)e * ncells + c] = S.Js[e];
     group_sync();
   }
+  }
+}
+
+// reactionRateRhoY over a box (Source/PeleLMeX_Reactions.cpp:399-435 calling
+// Source/PeleLMeX_K.H:848-889): one group per cell, MKS arrays in Array4 layout; cells with
+// mask == -1 (covered) get zero rates, as the reference's covered-cell branch does.
+struct RateParams {
+  int lo0, lo1, lo2, nx, ny;
+  long ncells;
+  Arr rY, rH, T, out;
+  IArr mask;
+  int has_mask;
+  const unsigned char* blob;
+  int blob_bytes, neff, ninc;
+};
+
+template <int N, int R, int G>
+__global__ void __launch_bounds__(BLOCK) rate_kernel(RateParams P) {
+  constexpr int RPL = Cell<N, R, G>::RPL;
+  extern __shared__ __align__(16) unsigned char smem[];
+  {
+    const uint4* srcb = reinterpret_cast<const uint4*>(P.blob);
+    uint4* dst = reinterpret_cast<uint4*>(smem);
+    for (int b = threadIdx.x; b < P.blob_bytes / 16; b += blockDim.x) dst[b] = srcb[b];
+  }
+  __syncthreads();
+  const BlobLayout BL = blob_layout(N, R, P.neff, P.ninc);
+  Smem<N, R> S;
+  const int lane = threadIdx.x % G;
+  const int grp = threadIdx.x / G;
+  setup_views<N, R, G>(S, smem, BL, grp, nullptr);
+  const long stride = (long)gridDim.x * (blockDim.x / G);
+  for (long c = (long)blockIdx.x * (blockDim.x / G) + grp; c < P.ncells; c += stride) {
+    const unsigned u = (unsigned)c;
+    const int i = P.lo0 + (int)(u % (unsigned)P.nx);
+    const int j = P.lo1 + (int)((u / (unsigned)P.nx) % (unsigned)P.ny);
+    const int k = P.lo2 + (int)(u / ((unsigned)P.nx * (unsigned)P.ny));
+    const long offO = aoff(P.out, i, j, k);
+    if (P.has_mask) {
+      const long mo = (long)(i - P.mask.bx) + (long)(j - P.mask.by) * P.mask.js + (long)(k - P.mask.bz) * P.mask.ks;
+      if (P.mask.p[mo] == -1) {
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) {
+          const int ii = lane + G * s;
+          if (ii < N) P.out.p[offO + (long)ii * P.out.ns] = 0.0;
+        }
+        continue;
+      }
+    }
+    const long offY = aoff(P.rY, i, j, k);
+    double ry[RPL], out[RPL];
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      const int ii = lane + G * s;
+      ry[s] = (ii < N) ? P.rY.p[offY + (long)ii * P.rY.ns] : 0.0;
+    }
+    rate_rhoY_eval<N, R, G>(S, lane, ry, P.rH.p[aoff(P.rH, i, j, k)], P.T.p[aoff(P.T, i, j, k)], out);
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      const int ii = lane + G * s;
+      if (ii < N) P.out.p[offO + (long)ii * P.out.ns] = out[s];
+    }
   }
 }
 

@@ -54,6 +54,7 @@ using EvalLaunch = hipError_t (*)(int mode, const unsigned char* blob, int blob_
                                   double t, double* out, double* gscr, int blocks, int block, size_t shmem,
                                   hipStream_t st);
 using Occ = hipError_t (*)(int* nb, int block, size_t shmem);
+using RateLaunch = hipError_t (*)(const prk::RateParams&, int blocks, int block, size_t shmem, hipStream_t);
 
 template <int N, int R, int G>
 hipError_t launch_t(const prk::KParams& P, int blocks, int block, size_t shmem, hipStream_t st) {
@@ -71,6 +72,12 @@ hipError_t eval_t(int mode, const unsigned char* blob, int blob_bytes, int neff,
   else
     hipLaunchKernelGGL((prk::eval_kernel<N, R, G, 1>), dim3(blocks), dim3(block), shmem, st, blob, blob_bytes, neff,
                        ninc, ncells, y, rYsrc, rhoh0, rhohsrc, t, out, gscr);
+  return hipGetLastError();
+}
+
+template <int N, int R, int G>
+hipError_t rate_t(const prk::RateParams& P, int blocks, int block, size_t shmem, hipStream_t st) {
+  hipLaunchKernelGGL((prk::rate_kernel<N, R, G>), dim3(blocks), dim3(block), shmem, st, P);
   return hipGetLastError();
 }
 
@@ -93,11 +100,12 @@ struct Variant {
   Occ occ;
   int scratch;  // LDS bytes per cell group
   int slot;     // global scratch bytes per cell group
+  RateLaunch rate;
 };
 
 #define PR_VARIANT(NAME, NS, NR, G) \
   {NAME, G, &launch_t<NS, NR, G>, &eval_t<NS, NR, G>, &occ_t<NS, NR, G>, scratch_bytes<NS, NR, G>(), \
-   prk::Cell<NS, NR, G>::gslot_doubles * 8}
+   prk::Cell<NS, NR, G>::gslot_doubles * 8, &rate_t<NS, NR, G>}
 
 const Variant kVariants[] = {
     PR_VARIANT("LiDryer", PR_LIDRYER_NSP, PR_LIDRYER_NR, 16),
@@ -516,6 +524,57 @@ int pr_rhs_batch(pr_handle* h, int64_t ncells, const double* y, const double* rY
 int pr_jac_batch(pr_handle* h, int64_t ncells, const double* y, const double* rYsrc, const double* rhohsrc,
                  double* J, void* stream) {
   return eval_batch(h, 1, ncells, y, rYsrc, nullptr, rhohsrc, 0.0, J, stream);
+}
+
+int pr_reaction_rate_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY,
+                         const pr_array4* rH, const pr_array4* T, const pr_array4* rYdot,
+                         const pr_iarray4* mask, void* stream) {
+  if (!h || !lo || !hi) return fail(PR_ERR_ARG, "NULL argument");
+  const int N = h->mech->nsp;
+  if (!arr_ok(rY, N) || !arr_ok(rH, 1) || !arr_ok(T, 1) || !arr_ok(rYdot, N)) return fail(PR_ERR_ARG, "missing array");
+  if ((rY->ncomp && rY->ncomp < N) || (rYdot->ncomp && rYdot->ncomp < N))
+    return fail(PR_ERR_ARG, "rY / rYdot need NUM_SPECIES components");
+  const long nx = (long)hi[0] - lo[0] + 1, ny = (long)hi[1] - lo[1] + 1, nz = (long)hi[2] - lo[2] + 1;
+  if (nx <= 0 || ny <= 0 || nz <= 0) return fail(PR_ERR_ARG, "empty box");
+  const long ncells = nx * ny * nz;
+  if (ncells > 0x7fffffffL) return fail(PR_ERR_ARG, "box too large");
+  HIPCHK(hipSetDevice(h->device));
+  prk::RateParams P{};
+  P.lo0 = lo[0];
+  P.lo1 = lo[1];
+  P.lo2 = lo[2];
+  P.nx = (int)nx;
+  P.ny = (int)ny;
+  P.ncells = ncells;
+  P.rY = to_arr(rY);
+  P.rH = to_arr(rH);
+  P.T = to_arr(T);
+  P.out = to_arr(rYdot);
+  P.has_mask = mask && mask->p;
+  if (P.has_mask) {
+    P.mask.p = mask->p;
+    P.mask.js = mask->jstride;
+    P.mask.ks = mask->kstride;
+    P.mask.bx = mask->begin[0];
+    P.mask.by = mask->begin[1];
+    P.mask.bz = mask->begin[2];
+  }
+  P.blob = h->d_blob;
+  P.blob_bytes = h->blob_bytes;
+  P.neff = h->mech->neff;
+  P.ninc = h->mech->ninc;
+  const int G = h->var->G;
+  const int block = pick_block(h);
+  const size_t shmem = (size_t)h->blob_bytes + (size_t)(block / G) * h->var->scratch;
+  int dev_cus = 0;
+  HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, h->device));
+  const long want = (ncells + (block / G) - 1) / (block / G);
+  const long cap = 8L * dev_cus;
+  const int blocks = (int)(want < cap ? want : cap);
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(h->var->rate(P, blocks, block, shmem, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
 }
 
 int pr_get_stats(const pr_handle* h, pr_stats* out) {

@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(_HERE, "libpelelmex_amd.so")
 _LIB = None
 
 EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_rhs_batch", "pr_jac_batch",
-           "pr_get_stats", "pr_num_species", "pr_destroy", "pr_last_error"]
+           "pr_reaction_rate_box", "pr_get_stats", "pr_num_species", "pr_destroy", "pr_last_error"]
 
 
 class Array4(C.Structure):
@@ -66,6 +66,10 @@ def lib():
                                    C.c_double, C.c_void_p, C.c_void_p]
         L.pr_jac_batch.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_void_p]
+        L.pr_reaction_rate_box.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(Array4),
+                                           C.POINTER(Array4), C.POINTER(Array4), C.POINTER(Array4),
+                                           C.POINTER(IArray4), C.c_void_p]
+        L.pr_reaction_rate_box.restype = C.c_int
         L.pr_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.pr_num_species.argtypes = [C.c_void_p]
         L.pr_destroy.argtypes = [C.c_void_p]
@@ -172,6 +176,24 @@ class Reactor:
         if rc != 0:
             raise ReactorError(lib().pr_last_error().decode())
         return J
+
+    def reaction_rate_rhoY(self, rY, rH, T, mask=None, rYdot=None, stream=None):
+        """reactionRateRhoY over a box of device tensors (MKS: rY [N, nz, ny, nx], rH and T
+        [nz, ny, nx]); returns rYdot [N, nz, ny, nx] (zero where mask == -1)."""
+        import torch
+        nz, ny, nx = T.shape
+        if rYdot is None:
+            rYdot = torch.empty_like(rY)
+        lo = (C.c_int * 3)(0, 0, 0)
+        hi = (C.c_int * 3)(nx - 1, ny - 1, nz - 1)
+        a = [_array4(rY, self.nsp), _array4(rH, 1), _array4(T, 1), _array4(rYdot, self.nsp)]
+        m = IArray4(mask.data_ptr(), nx, nx * ny, (C.c_int * 3)(0, 0, 0)) if mask is not None else None
+        st = stream if stream is not None else torch.cuda.current_stream()
+        rc = lib().pr_reaction_rate_box(self._h, lo, hi, *[C.byref(x) for x in a],
+                                        C.byref(m) if m is not None else None, C.c_void_p(st.cuda_stream))
+        if rc != 0:
+            raise ReactorError(f"pr_reaction_rate_box failed ({rc}): {lib().pr_last_error().decode()}")
+        return rYdot
 
     def stats(self):
         s = Stats()

@@ -156,3 +156,32 @@ def test_rhs_and_jacobian_bits(gpu, mech_box, lanes):
     r.close()
     _assert_bits("rhs", f, fref)
     _assert_bits("jac", J, Jref)
+
+
+@pytest.mark.parametrize("mech_box,lanes", [("h2", 16), ("ch4", 32), ("ch4", 64)])
+def test_reaction_rate_box_bits(gpu, mech_box, lanes):
+    """pr_reaction_rate_box (reactionRateRhoY over a box, MKS) bit for bit against the oracle;
+    covered cells (mask == -1) get zero rates, as the reference's covered-cell branch writes."""
+    import torch
+    from oracle import pyoracle as po
+    from pelelmex_amd.reactor import Reactor
+    box = W.h2_air_box(nx=16, ny=8, nz=4) if mech_box == "h2" else W.ch4_air_flame_box(nx=16, ny=8, nz=4)
+    nx, ny, nz = box["dims"]
+    nc = nx * ny * nz
+    m = po.Mech(box["mech"])
+    rY, rH = box["rY"] * 1e3, box["rH"] * 0.1
+    T0 = box["T"] + 25.0 * np.sin(np.arange(nc))
+    ref, _ = m.rate_rhoY(rY, rH, T0)
+    mask = np.ones(nc, dtype=np.int32)
+    mask[::7] = -1
+    ref[:, mask == -1] = 0.0
+    r = Reactor.create("ReactorCvode", box["mech"], dict(KEYS, **{"hipbdf.lanes_per_cell": lanes}))
+    r.init(2, 1)
+    dev = lambda a, *shape: torch.as_tensor(np.ascontiguousarray(a)).reshape(*shape).to(gpu).contiguous()
+    N = m.nsp
+    out = torch.full((N, nz, ny, nx), float("nan"), dtype=torch.float64, device=gpu)
+    r.reaction_rate_rhoY(dev(rY, N, nz, ny, nx), dev(rH, nz, ny, nx), dev(T0, nz, ny, nx),
+                         mask=dev(mask, nz, ny, nx), rYdot=out)
+    got = out.cpu().numpy().reshape(N, nc)
+    r.close()
+    _assert_bits("rYdot", got, ref)

@@ -288,3 +288,31 @@ def test_react_invariants_and_fixed_point():
                 None, 1e-4)
     assert o["status"][0] == 0
     assert abs(o["T"][0] - r[1]) < 1e-3
+
+
+@pytest.mark.parametrize("name", ["h2", "ch4"])
+def test_reaction_rate_rhoY_pins(name):
+    """reactionRateRhoY restatement (Source/PeleLMeX_K.H:848-889), MKS: its temperature solves
+    h(T, Y) = rhoH / rho, its rates conserve mass and elements, and, converted to CGS, they are
+    the reactor right-hand side without sources at that temperature (which the Jacobian, PMF
+    and fixture pins above hold), so a wrong unit factor, sign or weight fails here."""
+    box = W.h2_air_box(nx=8, ny=2, nz=2) if name == "h2" else W.ch4_air_flame_box(nx=16, ny=2, nz=2)
+    m = po.Mech(box["mech"])
+    mech = load(box["mech"])
+    nc = box["T"].size
+    rY, rH = box["rY"] * 1e3, box["rH"] * 0.1          # CGS -> MKS
+    T0 = box["T"] + 25.0 * np.sin(np.arange(nc))       # HY2T starts away from the answer
+    out, T = m.rate_rhoY(rY, rH, T0)
+    rho = rY.sum(0)
+    Y = rY / rho
+    np.testing.assert_allclose(mech.h_mass(T, Y.T) * 1e-4, rH / rho, rtol=1e-11)
+    np.testing.assert_allclose(T, box["T"], rtol=1e-8)
+    scale = np.abs(out).sum(0)
+    assert np.all(np.abs(out.sum(0)) <= 1e-13 * scale)
+    elem = mech.elem_matrix.T @ (out / m.W[:, None])
+    assert np.all(np.abs(elem) <= 1e-12 * (np.abs(out) / m.W[:, None]).sum(0))
+    assert np.any(scale > 0)
+    for c in range(nc):
+        cell = m.cell(box["rH"][c], 0.0, np.zeros(m.nsp))
+        f = m.rhs(cell, 0.0, np.append(box["rY"][:, c], T[c]), frozen=True)
+        np.testing.assert_allclose(out[:, c] * 1e-3, f[:-1], rtol=1e-12, atol=1e-12 * np.abs(f[:-1]).max())
