@@ -25,6 +25,9 @@
  *   pr_reaction_rate_box  reactionRateRhoY over a box (Source/PeleLMeX_K.H:848-889), as
  *                    PeleLM::computeInstantaneousReactionRate runs it per box
  *                    (Source/PeleLMeX_Reactions.cpp:399-435)
+ *   pr_surrogate_load, pr_surrogate_features, pr_mlp_*   no reference counterpart: the learned
+ *                    dt_react map that pr_react_box tries before the BDF (cells failing its FP64
+ *                    trust check are integrated by the BDF), and the kernels that train it
  */
 #ifndef PELE_REACTOR_CAPI_H
 #define PELE_REACTOR_CAPI_H
@@ -51,11 +54,17 @@ typedef struct {
 } pr_iarray4;
 
 typedef struct {
-  int64_t cells;      /* cells integrated (mask != -1) */
+  int64_t cells;      /* cells integrated by the BDF (mask != -1; with a surrogate: its rejects) */
   int64_t failed;     /* cells whose integration failed */
-  int64_t nst, nfe, nje, nsetups, nni, netf, ncfn; /* sums over cells */
-  double kernel_ms;   /* device time of the integration kernel */
+  int64_t nst, nfe, nje, nsetups, nni, netf, ncfn; /* sums over BDF cells */
+  double kernel_ms;   /* device time of the BDF integration kernel */
   int lanes_per_cell, blocks;
+  /* surrogate path (zero without one) */
+  int64_t sur_accepted, sur_rejected, sur_audited, audit_failed;
+  double total_ms;      /* device time of the whole react call */
+  double surrogate_ms;  /* features + network + trust check */
+  double audit_ms;      /* BDF re-solve of the audit sample */
+  double audit_max_dT_rel, audit_max_dY;  /* worst |dT| / T and |dY_k| of audited cells vs BDF */
 } pr_stats;
 
 /* Error codes (negative); pr_react_box returns >= 0 (number of failed cells) on success. */
@@ -94,6 +103,44 @@ int pr_jac_batch(pr_handle* h, int64_t ncells, const double* y, const double* rY
 int pr_reaction_rate_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY,
                          const pr_array4* rH, const pr_array4* T, const pr_array4* rYdot,
                          const pr_iarray4* mask, void* stream);
+/* Learned dt_react map. Network: nlayers dense layers, dims[0] = NUM_SPECIES + 2 inputs
+   (log10(Y_k + 1e-20), T / 2000, log10(dt)), dims[nlayers] = NUM_SPECIES + 1 outputs (dY_k, dT),
+   GELU between layers. params (host, fp32): for each layer W [dims[l] x dims[l+1]] row-major then
+   b [dims[l+1]]. xnorm: mean then standard deviation of the inputs (2 x dims[0]); ynorm: same
+   for the outputs (2 x dims[nlayers]). Once loaded (and surrogate.enable != 0) pr_react_box
+   evaluates it in BF16 on the matrix cores, accepts the cells that pass the FP64 trust check
+   (element balance, backward-Euler residual with the analytic rates at surrogate.trust_rtol /
+   surrogate.trust_atol) and integrates the others with the BDF; surrogate.audit_frac of the
+   accepted cells are re-solved by the BDF and their worst error reported in pr_stats. */
+int pr_surrogate_load(pr_handle* h, int nlayers, const int* dims, const float* params, const float* xnorm,
+                      const float* ynorm);
+/* Network inputs for a batch of states (device, component-major rY[k * ncells + c], T[c],
+   dt[c]): raw (fp32 [ncells x (N + 2)], may be NULL) and normalised bf16 X (may be NULL; needs
+   a loaded network). */
+int pr_surrogate_features(pr_handle* h, int64_t ncells, const double* rY, const double* T, const double* dt,
+                          float* raw, void* X, void* stream);
+/* Training kernels (device pointers; bf16 arrays as void*; all enqueue on stream):
+   C = op(A) op(B) (+ bias, then GELU when gelu != 0) with A(m, k) = A[m sam + k sak] and
+   B(k, n) = B[k sbk + n sbn] in bf16, fp32 accumulation; outputs C (fp32, ldc), Cb (bf16,
+   ldcb), Z (pre-activation fp32, ldz), each nullable; accumulate != 0 adds the raw product
+   into C instead. */
+int pr_mlp_gemm(int M, int N, int K, const void* A, int64_t sam, int64_t sak, const void* B, int64_t sbk,
+                int64_t sbn, const float* bias, int gelu, float* C, int64_t ldc, void* Cb, int64_t ldcb, float* Z,
+                int64_t ldz, int accumulate, void* stream);
+/* dZ = dA * gelu'(Z) into bf16 dZb (and fp32 dZ when non-NULL). */
+int pr_mlp_gelu_backward(int64_t rows, int cols, const float* dA, int64_t lda, const float* Z, int64_t ldz,
+                         void* dZb, int64_t ldb, float* dZ, int64_t ldd, void* stream);
+/* out[c] (+)= sum over rows of d[r * ld + c]. */
+int pr_mlp_column_sum(int64_t rows, int cols, const float* d, int64_t ld, float* out, int accumulate, void* stream);
+/* Mean-squared error: dY = 2 (Y - T) / (rows cols), *loss += mean of (Y - T)^2. */
+int pr_mlp_mse(int64_t rows, int cols, const float* Y, int64_t ldy, const float* T, int64_t ldt, float* dY,
+               float* loss, void* stream);
+/* Adam step `step` (1-based) on fp32 parameters p with gradient g and moments m, v; pb (bf16
+   copy, nullable) is refreshed. */
+int pr_mlp_adam(int64_t n, float* p, const float* g, float* m, float* v, void* pb, float lr, float beta1, float beta2,
+                float eps, int step, void* stream);
+/* b = bf16(a). */
+int pr_mlp_to_bf16(int64_t n, const float* a, void* b, void* stream);
 int pr_get_stats(const pr_handle* h, pr_stats* out);
 int pr_num_species(const pr_handle* h);
 void pr_destroy(pr_handle* h);

@@ -64,6 +64,8 @@ struct KParams {
   int* status;                   // per cell (linear box index), nullable
   unsigned int* nfail;
   double* gscr;                  // per-group global scratch slots (saved Jacobian, Jacobian records)
+  const int* cell_list;          // when set: queue entry q integrates box cell cell_list[q]
+  const unsigned int* ncells_dev;  // when set: min(*ncells_dev, ncells) entries are integrated
 };
 
 enum { ST_OK = 0, ST_TOO_MUCH_WORK = -1, ST_ERR_FAILURE = -3, ST_CONV_FAILURE = -4, ST_GUARD = -99 };

@@ -94,6 +94,11 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
   Ctl& c = *reinterpret_cast<Ctl*>(setup_views<N, R, G>(S, smem, BL, grp, P.gscr));
 
   const double tout = P.dt;
+  long nwork = P.ncells;
+  if (P.ncells_dev) {
+    const long d = (long)*P.ncells_dev;
+    nwork = d < nwork ? d : nwork;
+  }
   const int qmax = P.max_order < 1 ? 1 : (P.max_order > Q ? Q : P.max_order);
 
   c.s_cells = c.s_nst = c.s_nfe = c.s_nje = c.s_nset = c.s_nni = c.s_netf = c.s_ncfn = c.s_fail = 0u;
@@ -372,7 +377,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
       unsigned int cl = 0;
       if (lane == 0) cl = atomicAdd(P.queue, 1u);
       cl = __shfl(cl, 0, G);
-      if ((long)cl >= P.ncells) break;
+      if ((long)cl >= nwork) break;
+      if (P.cell_list) cl = (unsigned int)P.cell_list[cl];
       int i, j, k;
       cell_ijk((int)cl, i, j, k);
       c.cell = (int)cl;

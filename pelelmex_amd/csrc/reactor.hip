@@ -11,12 +11,13 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <sstream>
 #include <string>
 #include <vector>
 
 #include "../../include/pele_reactor_capi.h"
-#include "react_kernel.h"
+#include "surrogate_kernel.h"
 #include "mech/drm19.h"
 #include "mech/lidryer.h"
 
@@ -55,6 +56,8 @@ using EvalLaunch = hipError_t (*)(int mode, const unsigned char* blob, int blob_
                                   hipStream_t st);
 using Occ = hipError_t (*)(int* nb, int block, size_t shmem);
 using RateLaunch = hipError_t (*)(const prk::RateParams&, int blocks, int block, size_t shmem, hipStream_t);
+using TrustLaunch = hipError_t (*)(const prk::TrustParams&, int blocks, int block, size_t shmem, hipStream_t);
+using FeatLaunch = hipError_t (*)(const prk::FeatParams&, hipStream_t);
 
 template <int N, int R, int G>
 hipError_t launch_t(const prk::KParams& P, int blocks, int block, size_t shmem, hipStream_t st) {
@@ -82,6 +85,19 @@ hipError_t rate_t(const prk::RateParams& P, int blocks, int block, size_t shmem,
 }
 
 template <int N, int R, int G>
+hipError_t trust_t(const prk::TrustParams& P, int blocks, int block, size_t shmem, hipStream_t st) {
+  hipLaunchKernelGGL((prk::trust_kernel<N, R, G>), dim3(blocks), dim3(block), shmem, st, P);
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t feat_t(const prk::FeatParams& P, hipStream_t st) {
+  if (P.ncells <= 0) return hipSuccess;
+  hipLaunchKernelGGL((prk::featurize_kernel<N>), dim3((unsigned)((P.ncells + 127) / 128)), dim3(128), 0, st, P);
+  return hipGetLastError();
+}
+
+template <int N, int R, int G>
 hipError_t occ_t(int* nb, int block, size_t shmem) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(
       nb, reinterpret_cast<const void*>(&prk::react_kernel<N, R, G>), block, shmem);
@@ -101,11 +117,13 @@ struct Variant {
   int scratch;  // LDS bytes per cell group
   int slot;     // global scratch bytes per cell group
   RateLaunch rate;
+  TrustLaunch trust;
+  FeatLaunch feat;
 };
 
 #define PR_VARIANT(NAME, NS, NR, G) \
   {NAME, G, &launch_t<NS, NR, G>, &eval_t<NS, NR, G>, &occ_t<NS, NR, G>, scratch_bytes<NS, NR, G>(), \
-   prk::Cell<NS, NR, G>::gslot_doubles * 8, &rate_t<NS, NR, G>}
+   prk::Cell<NS, NR, G>::gslot_doubles * 8, &rate_t<NS, NR, G>, &trust_t<NS, NR, G>, &feat_t<NS>}
 
 const Variant kVariants[] = {
     PR_VARIANT("LiDryer", PR_LIDRYER_NSP, PR_LIDRYER_NR, 16),
@@ -202,6 +220,10 @@ struct Keys {
   std::string solve_type = "GMRES";
   int clean_init_massfrac = 0, analytical_jacobian = 1, verbose = 0;
   int lanes_per_cell = 0, block = 0;
+  // learned map in front of the BDF (used once weights are loaded with pr_surrogate_load)
+  int surr_enable = 1;
+  double surr_trust_rtol = 1e-3, surr_trust_atol = 1e-9, surr_elem_tol = 0.05, surr_audit_frac = 0.01;
+  int surr_audit_seed = 12345;
 };
 
 int parse_keys(const char* text, Keys& K) {
@@ -266,7 +288,26 @@ int parse_keys(const char* text, Keys& K) {
     } else if (k == "hipbdf.block") {
       if (!num(d) || ((int)d % 64) != 0 || d < 64 || d > 256) return fail(PR_ERR_KEY, "hipbdf.block must be 64, 128, 192 or 256");
       K.block = (int)d;
-    } else if (k.rfind("ode.", 0) == 0 || k.rfind("cvode.", 0) == 0 || k.rfind("hipbdf.", 0) == 0) {
+    } else if (k == "surrogate.enable") {
+      if (!num(d)) return fail(PR_ERR_KEY, "bad surrogate.enable");
+      K.surr_enable = (int)d;
+    } else if (k == "surrogate.trust_rtol") {
+      if (!num(d) || !(d > 0)) return fail(PR_ERR_KEY, "bad surrogate.trust_rtol");
+      K.surr_trust_rtol = d;
+    } else if (k == "surrogate.trust_atol") {
+      if (!num(d) || !(d > 0)) return fail(PR_ERR_KEY, "bad surrogate.trust_atol");
+      K.surr_trust_atol = d;
+    } else if (k == "surrogate.elem_tol") {
+      if (!num(d) || !(d >= 0)) return fail(PR_ERR_KEY, "bad surrogate.elem_tol");
+      K.surr_elem_tol = d;
+    } else if (k == "surrogate.audit_frac") {
+      if (!num(d) || d < 0 || d > 1) return fail(PR_ERR_KEY, "surrogate.audit_frac must be in [0, 1]");
+      K.surr_audit_frac = d;
+    } else if (k == "surrogate.audit_seed") {
+      if (!num(d)) return fail(PR_ERR_KEY, "bad surrogate.audit_seed");
+      K.surr_audit_seed = (int)d;
+    } else if (k.rfind("ode.", 0) == 0 || k.rfind("cvode.", 0) == 0 || k.rfind("hipbdf.", 0) == 0 ||
+               k.rfind("surrogate.", 0) == 0) {
       return fail(PR_ERR_KEY, "unknown reactor key " + k);
     }
     // keys of other namespaces are not the reactor's and are left alone
@@ -275,6 +316,25 @@ int parse_keys(const char* text, Keys& K) {
 }
 
 }  // namespace
+
+// Loaded network and the per-box buffers of the surrogate path.
+struct Surrogate {
+  std::vector<int> dims;            // layer widths, dims[0] = N + 2, back = N + 1
+  std::vector<__bf16*> W;           // [dims[l] x dims[l+1]] bf16
+  std::vector<float*> b;            // [dims[l+1]]
+  float *xmu = nullptr, *xsd = nullptr, *ymu = nullptr, *ysd = nullptr;
+  double *Mel = nullptr, *Minv = nullptr;
+  int E = 0;
+  long cap = 0;                     // cells the buffers hold
+  __bf16 *X = nullptr, *H0 = nullptr, *H1 = nullptr;
+  float* Yout = nullptr;
+  int* rej = nullptr;
+  unsigned int* cnt = nullptr;      // [0] rejected [1] audited [2] accepted [3] fallback queue [4] audit queue
+  unsigned int aud_cap = 0;
+  double *aud_in = nullptr, *aud_out = nullptr, *aud_box = nullptr, *aud_err = nullptr;
+  int* aud_cell = nullptr;
+  std::vector<void*> owned;
+};
 
 struct pr_handle {
   const pr_mech_t* mech = nullptr;
@@ -285,12 +345,13 @@ struct pr_handle {
   unsigned char* d_blob = nullptr;
   int blob_bytes = 0;
   double* d_atolv = nullptr;
-  void* d_counters = nullptr;  // queue (4 B), nfail (4 B), stats (8 x 8 B)
+  void* d_counters = nullptr;  // two regions of 128 B: queue (4 B), nfail (4 B), stats (8 x 8 B at +64)
   double* d_gscr = nullptr;    // per-group global scratch slots
   size_t gscr_bytes = 0;
   bool use_typ = false;
   pr_stats last{};
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  Surrogate* surr = nullptr;
 };
 
 extern "C" {
@@ -329,10 +390,11 @@ int pr_create(const char* integrator, const char* mechanism, int reactor_type, i
   }
   std::vector<unsigned char> blob = pack_blob(m);
   h->blob_bytes = (int)blob.size();
-  if (hipMalloc(&h->d_blob, blob.size()) != hipSuccess || hipMalloc(&h->d_counters, 128) != hipSuccess ||
+  if (hipMalloc(&h->d_blob, blob.size()) != hipSuccess || hipMalloc(&h->d_counters, 256) != hipSuccess ||
       hipMalloc(&h->d_atolv, sizeof(double) * (m->nsp + 1)) != hipSuccess ||
       hipMemcpy(h->d_blob, blob.data(), blob.size(), hipMemcpyHostToDevice) != hipSuccess ||
-      hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+      hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
+      hipEventCreate(&h->ev2) != hipSuccess || hipEventCreate(&h->ev3) != hipSuccess) {
     pr_destroy(h);
     return fail(PR_ERR_HIP, "device allocation failed");
   }
@@ -387,6 +449,117 @@ static prk::Arr to_arr(const pr_array4* a) {
   r.by = a->begin[1];
   r.bz = a->begin[2];
   return r;
+}
+
+// Launches the persistent BDF kernel for up to work_max queue entries; zeroes P.queue, P.stats
+// and P.nfail first (all on the stream).
+static int bdf_launch(pr_handle* h, prk::KParams& P, long work_max, hipStream_t st) {
+  const int G = h->var->G;
+  const int block = pick_block(h);
+  const size_t shmem = (size_t)h->blob_bytes + (size_t)(block / G) * h->var->scratch;
+  int dev_cus = 0, nb = 0;
+  HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, h->device));
+  HIPCHK(h->var->occ(&nb, block, shmem));
+  if (nb < 1) return fail(PR_ERR_HIP, "kernel does not fit on a CU with this block / LDS size");
+  const long cap = (long)nb * dev_cus;
+  long want = (work_max + (block / G) - 1) / (block / G);
+  if (want < 1) want = 1;
+  const int blocks = (int)(want < cap ? want : cap);
+  {
+    int rc = ensure_gscr(h, (long)blocks * (block / G));
+    if (rc) return rc;
+  }
+  P.gscr = h->d_gscr;
+  hipLaunchKernelGGL(prk::zero_counters, dim3(1), dim3(64), 0, st, P.queue, P.stats, P.nfail);
+  HIPCHK(hipGetLastError());
+  HIPCHK(h->var->launch(P, blocks, block, shmem, st));
+  h->last.lanes_per_cell = G;
+  h->last.blocks = blocks;
+  return 0;
+}
+
+static int surr_alloc(Surrogate* S, void** p, size_t bytes) {
+  HIPCHK(hipMalloc(p, bytes));
+  S->owned.push_back(*p);
+  return 0;
+}
+
+// Buffers of the surrogate path for boxes of up to ncells cells.
+static int ensure_surr_buffers(pr_handle* h, long ncells) {
+  Surrogate* S = h->surr;
+  if (ncells <= S->cap) return 0;
+  const int N = h->mech->nsp;
+  for (void* q : {(void*)S->X, (void*)S->H0, (void*)S->H1, (void*)S->Yout, (void*)S->rej, (void*)S->aud_in,
+                  (void*)S->aud_out, (void*)S->aud_box, (void*)S->aud_err, (void*)S->aud_cell})
+    if (q) {
+      (void)hipFree(q);
+      for (auto it = S->owned.begin(); it != S->owned.end(); ++it)
+        if (*it == q) {
+          S->owned.erase(it);
+          break;
+        }
+    }
+  S->cap = 0;
+  int hid = 1;
+  for (size_t l = 1; l + 1 < S->dims.size(); ++l) hid = S->dims[l] > hid ? S->dims[l] : hid;
+  const double frac = h->keys.surr_audit_frac;
+  const long acap = frac > 0 ? (long)(ncells * frac * 1.5) + 64 : 0;
+  S->aud_cap = (unsigned int)(acap < ncells ? acap : ncells);
+  const long ac = S->aud_cap > 0 ? S->aud_cap : 1;
+  int rc = 0;
+  if ((rc = surr_alloc(S, (void**)&S->X, sizeof(__bf16) * ncells * (N + 2))) ||
+      (rc = surr_alloc(S, (void**)&S->H0, sizeof(__bf16) * ncells * hid)) ||
+      (rc = surr_alloc(S, (void**)&S->H1, sizeof(__bf16) * ncells * hid)) ||
+      (rc = surr_alloc(S, (void**)&S->Yout, sizeof(float) * ncells * (N + 1))) ||
+      (rc = surr_alloc(S, (void**)&S->rej, sizeof(int) * ncells)) ||
+      (rc = surr_alloc(S, (void**)&S->aud_in, sizeof(double) * ac * (2 * N + 3))) ||
+      (rc = surr_alloc(S, (void**)&S->aud_out, sizeof(double) * ac * (N + 1))) ||
+      (rc = surr_alloc(S, (void**)&S->aud_box, sizeof(double) * ac * (2 * N + 4))) ||
+      (rc = surr_alloc(S, (void**)&S->aud_err, sizeof(double) * 2 * ac)) ||
+      (rc = surr_alloc(S, (void**)&S->aud_cell, sizeof(int) * ac)))
+    return rc;
+  S->cap = ncells;
+  return 0;
+}
+
+static hipError_t gemm_launch(const prk::GemmParams& G, hipStream_t st) {
+  if (G.M <= 0 || G.N <= 0) return hipSuccess;
+  dim3 grid((unsigned)((G.N + prk::GT_N - 1) / prk::GT_N), (unsigned)((G.M + prk::GT_M - 1) / prk::GT_M));
+  hipLaunchKernelGGL(prk::gemm_bf16_kernel, grid, dim3(256), 0, st, G);
+  return hipGetLastError();
+}
+
+// Network forward over ncells feature rows S->X: hidden layers GELU into bf16, last layer fp32.
+static int mlp_forward(pr_handle* h, long ncells, hipStream_t st) {
+  Surrogate* S = h->surr;
+  const int nl = (int)S->dims.size() - 1;
+  const __bf16* in = S->X;
+  for (int l = 0; l < nl; ++l) {
+    prk::GemmParams G{};
+    G.M = (int)ncells;
+    G.N = S->dims[l + 1];
+    G.K = S->dims[l];
+    G.A = in;
+    G.sam = G.K;
+    G.sak = 1;
+    G.B = S->W[l];
+    G.sbk = G.N;
+    G.sbn = 1;
+    G.bias = S->b[l];
+    const bool last = (l == nl - 1);
+    G.epi = last ? prk::EPI_NONE : prk::EPI_GELU;
+    __bf16* out = (l % 2 == 0) ? S->H0 : S->H1;
+    if (last) {
+      G.C = S->Yout;
+      G.ldc = G.N;
+    } else {
+      G.Cb = out;
+      G.ldcb = G.N;
+    }
+    HIPCHK(gemm_launch(G, st));
+    in = out;
+  }
+  return 0;
 }
 
 int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY,
@@ -444,36 +617,147 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   P.nfail = reinterpret_cast<unsigned int*>(base + 4);
   P.stats = reinterpret_cast<unsigned long long*>(base + 64);
   P.status = status_out;
+  h->last = pr_stats{};
 
-  const int G = h->var->G;
-  const int block = pick_block(h);
-  const size_t shmem = (size_t)h->blob_bytes + (size_t)(block / G) * h->var->scratch;
-  int dev_cus = 0, nb = 0;
-  HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, h->device));
-  HIPCHK(h->var->occ(&nb, block, shmem));
-  if (nb < 1) return fail(PR_ERR_HIP, "kernel does not fit on a CU with this block / LDS size");
-  const long groups_needed = ncells;
-  const long cap = (long)nb * dev_cus;
-  const long want = (groups_needed + (block / G) - 1) / (block / G);
-  const int blocks = (int)(want < cap ? want : cap);
+  Surrogate* S = (h->surr && h->keys.surr_enable) ? h->surr : nullptr;
+  HIPCHK(hipEventRecord(h->ev0, st));
+  if (S) {
+    int rc = ensure_surr_buffers(h, ncells);
+    if (rc) return rc;
+    prk::FeatParams F{};
+    F.lo0 = lo[0];
+    F.lo1 = lo[1];
+    F.lo2 = lo[2];
+    F.nx = (int)nx;
+    F.ny = (int)ny;
+    F.ncells = ncells;
+    F.rY = P.rY;
+    F.T = P.T;
+    F.dt = dt;
+    F.xmu = S->xmu;
+    F.xsd = S->xsd;
+    F.X = S->X;
+    HIPCHK(h->var->feat(F, st));
+    if ((rc = mlp_forward(h, ncells, st))) return rc;
+    HIPCHK(hipMemsetAsync(S->cnt, 0, 8 * sizeof(unsigned int), st));
+    prk::TrustParams Q{};
+    Q.lo0 = lo[0];
+    Q.lo1 = lo[1];
+    Q.lo2 = lo[2];
+    Q.nx = (int)nx;
+    Q.ny = (int)ny;
+    Q.ncells = ncells;
+    Q.rY = P.rY;
+    Q.rYsrc = P.rYsrc;
+    Q.T = P.T;
+    Q.rH = P.rH;
+    Q.rHsrc = P.rHsrc;
+    Q.FC = P.FC;
+    Q.mask = P.mask;
+    Q.has_mask = P.has_mask;
+    Q.status = status_out;
+    Q.dt = dt;
+    Q.Yout = S->Yout;
+    Q.ymu = S->ymu;
+    Q.ysd = S->ysd;
+    Q.Mel = S->Mel;
+    Q.Minv = S->Minv;
+    Q.E = S->E;
+    Q.trust_rtol = h->keys.surr_trust_rtol;
+    Q.trust_atol = h->keys.surr_trust_atol;
+    Q.elem_tol = h->keys.surr_elem_tol;
+    Q.blob = h->d_blob;
+    Q.blob_bytes = h->blob_bytes;
+    Q.neff = h->mech->neff;
+    Q.ninc = h->mech->ninc;
+    Q.rej_list = S->rej;
+    Q.counters = S->cnt;
+    Q.audit_cap = S->aud_cap;
+    const double thr = h->keys.surr_audit_frac * 4294967296.0;
+    Q.audit_thresh = thr >= 4294967295.0 ? 0xFFFFFFFFu : (unsigned int)thr;
+    Q.audit_seed = (unsigned int)h->keys.surr_audit_seed;
+    Q.aud_in = S->aud_in;
+    Q.aud_out = S->aud_out;
+    Q.aud_cell = S->aud_cell;
+    const int G = h->var->G;
+    const int block = pick_block(h);
+    const size_t shmem = (size_t)h->blob_bytes + (size_t)(block / G) * h->var->scratch;
+    int dev_cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    const long want = (ncells + (block / G) - 1) / (block / G);
+    const long cap = 8L * dev_cus;
+    HIPCHK(h->var->trust(Q, (int)(want < cap ? want : cap), block, shmem, st));
+    P.cell_list = S->rej;
+    P.ncells_dev = &S->cnt[0];
+  }
+  HIPCHK(hipEventRecord(h->ev1, st));
   {
-    int rc = ensure_gscr(h, (long)blocks * (block / G));
+    int rc = bdf_launch(h, P, ncells, st);
     if (rc) return rc;
   }
-  P.gscr = h->d_gscr;
-
-  hipLaunchKernelGGL(prk::zero_counters, dim3(1), dim3(64), 0, st, P.queue, P.stats, P.nfail);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(h->ev0, st));
-  HIPCHK(h->var->launch(P, blocks, block, shmem, st));
-  HIPCHK(hipEventRecord(h->ev1, st));
+  HIPCHK(hipEventRecord(h->ev2, st));
+  const unsigned int acap = S ? S->aud_cap : 0u;
+  if (S && acap > 0) {
+    // audit: the BDF re-solves the sampled accepted cells from their inputs, as a 1D batch of
+    // stride acap, with its own queue and counters (second half of d_counters)
+    const long c = acap;
+    double* a_rY = S->aud_box;
+    double* a_src = a_rY + (long)N * c;
+    double* a_T = a_src + (long)N * c;
+    double* a_rH = a_T + c;
+    double* a_rHsrc = a_rH + c;
+    double* a_FC = a_rHsrc + c;
+    const unsigned grid = (unsigned)((c + 127) / 128);
+    hipLaunchKernelGGL(prk::audit_unpack_kernel, dim3(grid), dim3(128), 0, st, N, &S->cnt[1], acap, S->aud_in, a_rY,
+                       a_src, a_T, a_rH, a_rHsrc, a_FC);
+    HIPCHK(hipGetLastError());
+    prk::KParams A = P;
+    A.lo0 = A.lo1 = A.lo2 = 0;
+    A.nx = (int)c;
+    A.ny = 1;
+    A.ncells = c;
+    auto arr1 = [&](double* p, long ns) {
+      prk::Arr r;
+      r.p = p;
+      r.js = c;
+      r.ks = c;
+      r.ns = ns;
+      r.bx = r.by = r.bz = 0;
+      return r;
+    };
+    A.rY = arr1(a_rY, c);
+    A.rYsrc = arr1(a_src, c);
+    A.T = arr1(a_T, 0);
+    A.rH = arr1(a_rH, 0);
+    A.rHsrc = arr1(a_rHsrc, 0);
+    A.FC = arr1(a_FC, 0);
+    A.has_mask = 0;
+    A.status = nullptr;
+    A.cell_list = nullptr;
+    A.ncells_dev = &S->cnt[1];
+    A.queue = reinterpret_cast<unsigned int*>(base + 128);
+    A.nfail = reinterpret_cast<unsigned int*>(base + 132);
+    A.stats = reinterpret_cast<unsigned long long*>(base + 192);
+    {
+      int rc = bdf_launch(h, A, c, st);
+      if (rc) return rc;
+    }
+    hipLaunchKernelGGL(prk::audit_err_kernel, dim3(grid), dim3(128), 0, st, N, &S->cnt[1], acap, a_rY, a_T,
+                       S->aud_out, S->aud_err);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipEventRecord(h->ev3, st));
   unsigned long long hs[8];
   unsigned int nfail = 0;
+  unsigned int sc[3] = {0u, 0u, 0u};
   HIPCHK(hipMemcpyAsync(hs, P.stats, sizeof(hs), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&nfail, P.nfail, sizeof(nfail), hipMemcpyDeviceToHost, st));
+  if (S) HIPCHK(hipMemcpyAsync(sc, S->cnt, sizeof(sc), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  float ms_all = 0.f, ms_sur = 0.f, ms_bdf = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms_all, h->ev0, h->ev3));
+  HIPCHK(hipEventElapsedTime(&ms_sur, h->ev0, h->ev1));
+  HIPCHK(hipEventElapsedTime(&ms_bdf, h->ev1, h->ev2));
   h->last.cells = (int64_t)hs[0];
   h->last.nst = (int64_t)hs[1];
   h->last.nfe = (int64_t)hs[2];
@@ -483,9 +767,29 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   h->last.netf = (int64_t)hs[6];
   h->last.ncfn = (int64_t)hs[7];
   h->last.failed = nfail;
-  h->last.kernel_ms = ms;
-  h->last.lanes_per_cell = G;
-  h->last.blocks = blocks;
+  h->last.kernel_ms = ms_bdf;
+  h->last.total_ms = ms_all;
+  h->last.surrogate_ms = ms_sur;
+  h->last.sur_accepted = sc[2];
+  h->last.sur_rejected = sc[0];
+  h->last.sur_audited = sc[1] < acap ? sc[1] : acap;
+  if (S && h->last.sur_audited > 0) {
+    float ms_aud = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms_aud, h->ev2, h->ev3));
+    h->last.audit_ms = ms_aud;
+    std::vector<double> err(2 * (size_t)h->last.sur_audited);
+    unsigned int afail = 0;
+    HIPCHK(hipMemcpy(err.data(), S->aud_err, sizeof(double) * err.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&afail, base + 132, sizeof(afail), hipMemcpyDeviceToHost));
+    double mT = 0.0, mY = 0.0;
+    for (size_t a = 0; a < err.size(); a += 2) {
+      mT = err[a] > mT || err[a] != err[a] ? err[a] : mT;
+      mY = err[a + 1] > mY || err[a + 1] != err[a + 1] ? err[a + 1] : mY;
+    }
+    h->last.audit_max_dT_rel = mT;
+    h->last.audit_max_dY = mY;
+    h->last.audit_failed = afail;
+  }
   if (time) *time += dt;
   return (int)nfail;
 }
@@ -577,6 +881,203 @@ int pr_reaction_rate_box(pr_handle* h, const int lo[3], const int hi[3], const p
   return 0;
 }
 
+int pr_surrogate_load(pr_handle* h, int nlayers, const int* dims, const float* params, const float* xnorm,
+                      const float* ynorm) {
+  if (!h || !dims || !params || !xnorm || !ynorm || nlayers < 1 || nlayers > 64) return fail(PR_ERR_ARG, "bad argument");
+  const int N = h->mech->nsp;
+  if (dims[0] != N + 2 || dims[nlayers] != N + 1)
+    return fail(PR_ERR_ARG, "surrogate needs NUM_SPECIES + 2 inputs and NUM_SPECIES + 1 outputs");
+  for (int l = 0; l <= nlayers; ++l)
+    if (dims[l] < 1 || dims[l] > 8192) return fail(PR_ERR_ARG, "layer width out of range");
+  for (int l = 0; l < 2 * dims[0]; ++l)
+    if (!std::isfinite(xnorm[l]) || (l >= dims[0] && !(xnorm[l] > 0))) return fail(PR_ERR_ARG, "bad xnorm");
+  for (int l = 0; l < 2 * dims[nlayers]; ++l)
+    if (!std::isfinite(ynorm[l]) || (l >= dims[nlayers] && !(ynorm[l] > 0))) return fail(PR_ERR_ARG, "bad ynorm");
+  // element matrix M_ek = a_ek / W_k and (M M^T)^-1 by Gauss-Jordan
+  const int E = h->mech->nelem;
+  if (E < 1 || E > prk::SURR_MAXE) return fail(PR_ERR_UNSUPPORTED, "too many elements for the trust check");
+  std::vector<double> Mel((size_t)E * N), MM((size_t)E * E, 0.0), Minv((size_t)E * E, 0.0);
+  for (int e = 0; e < E; ++e)
+    for (int k = 0; k < N; ++k) Mel[(size_t)e * N + k] = h->mech->elem[k * E + e] / h->mech->W[k];
+  for (int e = 0; e < E; ++e)
+    for (int f = 0; f < E; ++f) {
+      double a = 0.0;
+      for (int k = 0; k < N; ++k) a += Mel[(size_t)e * N + k] * Mel[(size_t)f * N + k];
+      MM[(size_t)e * E + f] = a;
+    }
+  for (int e = 0; e < E; ++e) Minv[(size_t)e * E + e] = 1.0;
+  for (int col = 0; col < E; ++col) {
+    int pr = col;
+    for (int r = col + 1; r < E; ++r)
+      if (std::fabs(MM[(size_t)r * E + col]) > std::fabs(MM[(size_t)pr * E + col])) pr = r;
+    if (!(std::fabs(MM[(size_t)pr * E + col]) > 0)) return fail(PR_ERR_UNSUPPORTED, "singular element matrix");
+    for (int j = 0; j < E; ++j) {
+      std::swap(MM[(size_t)col * E + j], MM[(size_t)pr * E + j]);
+      std::swap(Minv[(size_t)col * E + j], Minv[(size_t)pr * E + j]);
+    }
+    const double d = MM[(size_t)col * E + col];
+    for (int j = 0; j < E; ++j) {
+      MM[(size_t)col * E + j] /= d;
+      Minv[(size_t)col * E + j] /= d;
+    }
+    for (int r = 0; r < E; ++r)
+      if (r != col) {
+        const double f = MM[(size_t)r * E + col];
+        for (int j = 0; j < E; ++j) {
+          MM[(size_t)r * E + j] -= f * MM[(size_t)col * E + j];
+          Minv[(size_t)r * E + j] -= f * Minv[(size_t)col * E + j];
+        }
+      }
+  }
+  HIPCHK(hipSetDevice(h->device));
+  auto* S = new Surrogate();
+  auto cleanup = [&](int code, const char* msg) {
+    for (void* q : S->owned) (void)hipFree(q);
+    delete S;
+    return fail(code, msg);
+  };
+  S->dims.assign(dims, dims + nlayers + 1);
+  S->E = E;
+  size_t off = 0;
+  for (int l = 0; l < nlayers; ++l) {
+    const size_t nw = (size_t)dims[l] * dims[l + 1], nb = (size_t)dims[l + 1];
+    float* tmp = nullptr;
+    __bf16* w = nullptr;
+    float* b = nullptr;
+    if (surr_alloc(S, (void**)&w, nw * sizeof(__bf16)) || surr_alloc(S, (void**)&b, nb * sizeof(float)) ||
+        hipMalloc(&tmp, nw * sizeof(float)) != hipSuccess)
+      return cleanup(PR_ERR_HIP, "device allocation failed");
+    bool ok = hipMemcpy(tmp, params + off, nw * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(b, params + off + nw, nb * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+      hipLaunchKernelGGL(prk::f32_to_bf16_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, 0, (long)nw, tmp, w);
+      ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+    }
+    (void)hipFree(tmp);
+    if (!ok) return cleanup(PR_ERR_HIP, "weight upload failed");
+    S->W.push_back(w);
+    S->b.push_back(b);
+    off += nw + nb;
+  }
+  const int nin = dims[0], nout = dims[nlayers];
+  if (surr_alloc(S, (void**)&S->xmu, 2 * nin * sizeof(float)) || surr_alloc(S, (void**)&S->ymu, 2 * nout * sizeof(float)) ||
+      surr_alloc(S, (void**)&S->Mel, Mel.size() * sizeof(double)) ||
+      surr_alloc(S, (void**)&S->Minv, Minv.size() * sizeof(double)) ||
+      surr_alloc(S, (void**)&S->cnt, 8 * sizeof(unsigned int)))
+    return cleanup(PR_ERR_HIP, "device allocation failed");
+  S->xsd = S->xmu + nin;
+  S->ysd = S->ymu + nout;
+  if (hipMemcpy(S->xmu, xnorm, 2 * nin * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(S->ymu, ynorm, 2 * nout * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(S->Mel, Mel.data(), Mel.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(S->Minv, Minv.data(), Minv.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(PR_ERR_HIP, "upload failed");
+  if (h->surr) {
+    for (void* q : h->surr->owned) (void)hipFree(q);
+    delete h->surr;
+  }
+  h->surr = S;
+  return 0;
+}
+
+int pr_surrogate_features(pr_handle* h, int64_t ncells, const double* rY, const double* T, const double* dt,
+                          float* raw, void* X, void* stream) {
+  if (!h || !rY || !T || !dt || ncells <= 0 || (!raw && !X)) return fail(PR_ERR_ARG, "bad argument");
+  if (X && !h->surr) return fail(PR_ERR_ARG, "normalised features need a loaded surrogate");
+  HIPCHK(hipSetDevice(h->device));
+  prk::FeatParams F{};
+  F.ncells = ncells;
+  F.batch = 1;
+  F.rY.p = const_cast<double*>(rY);
+  F.T.p = const_cast<double*>(T);
+  F.dtv = dt;
+  F.xmu = X ? h->surr->xmu : nullptr;
+  F.xsd = X ? h->surr->xsd : nullptr;
+  F.raw = raw;
+  F.X = static_cast<__bf16*>(X);
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(h->var->feat(F, st));
+  return 0;
+}
+
+int pr_mlp_gemm(int M, int N, int K, const void* A, int64_t sam, int64_t sak, const void* B, int64_t sbk,
+                int64_t sbn, const float* bias, int gelu, float* C, int64_t ldc, void* Cb, int64_t ldcb, float* Z,
+                int64_t ldz, int accumulate, void* stream) {
+  if (M < 0 || N < 0 || K < 1 || !A || !B || (accumulate && !C) || (!C && !Cb && !Z))
+    return fail(PR_ERR_ARG, "bad gemm arguments");
+  if ((sak != 1 && sam != 1) || (sbk != 1 && sbn != 1)) return fail(PR_ERR_ARG, "each operand needs a unit stride");
+  prk::GemmParams G{};
+  G.M = M;
+  G.N = N;
+  G.K = K;
+  G.A = static_cast<const __bf16*>(A);
+  G.sam = sam;
+  G.sak = sak;
+  G.B = static_cast<const __bf16*>(B);
+  G.sbk = sbk;
+  G.sbn = sbn;
+  G.bias = bias;
+  G.epi = gelu ? prk::EPI_GELU : prk::EPI_NONE;
+  G.C = C;
+  G.ldc = ldc;
+  G.Cb = static_cast<__bf16*>(Cb);
+  G.ldcb = ldcb;
+  G.Z = Z;
+  G.ldz = ldz;
+  G.accumulate = accumulate;
+  HIPCHK(gemm_launch(G, (hipStream_t)stream));
+  return 0;
+}
+
+int pr_mlp_gelu_backward(int64_t rows, int cols, const float* dA, int64_t lda, const float* Z, int64_t ldz,
+                         void* dZb, int64_t ldb, float* dZ, int64_t ldd, void* stream) {
+  if (rows < 0 || cols < 1 || !dA || !Z || !dZb) return fail(PR_ERR_ARG, "bad argument");
+  const long n = (long)rows * cols;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(prk::gelu_bwd_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     (long)rows, cols, dA, (long)lda, Z, (long)ldz, static_cast<__bf16*>(dZb), (long)ldb, dZ, (long)ldd);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int pr_mlp_column_sum(int64_t rows, int cols, const float* d, int64_t ld, float* out, int accumulate, void* stream) {
+  if (rows < 0 || cols < 1 || !d || !out) return fail(PR_ERR_ARG, "bad argument");
+  hipLaunchKernelGGL(prk::col_sum_kernel, dim3((unsigned)((cols + 63) / 64)), dim3(256), 0, (hipStream_t)stream,
+                     (long)rows, cols, d, (long)ld, out, accumulate);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int pr_mlp_mse(int64_t rows, int cols, const float* Y, int64_t ldy, const float* T, int64_t ldt, float* dY,
+               float* loss, void* stream) {
+  if (rows < 1 || cols < 1 || !Y || !T || !dY || !loss) return fail(PR_ERR_ARG, "bad argument");
+  const long n = (long)rows * cols;
+  hipLaunchKernelGGL(prk::mse_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, (long)rows,
+                     cols, Y, (long)ldy, T, (long)ldt, dY, loss);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int pr_mlp_adam(int64_t n, float* p, const float* g, float* m, float* v, void* pb, float lr, float beta1, float beta2,
+                float eps, int step, void* stream) {
+  if (n < 0 || !p || !g || !m || !v || step < 1) return fail(PR_ERR_ARG, "bad argument");
+  if (n == 0) return 0;
+  const float bc1 = 1.0f - std::pow(beta1, (float)step), bc2 = 1.0f - std::pow(beta2, (float)step);
+  hipLaunchKernelGGL(prk::adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, (long)n,
+                     p, g, m, v, static_cast<__bf16*>(pb), lr, beta1, beta2, eps, bc1, bc2);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int pr_mlp_to_bf16(int64_t n, const float* a, void* b, void* stream) {
+  if (n < 0 || !a || !b) return fail(PR_ERR_ARG, "bad argument");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(prk::f32_to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     (long)n, a, static_cast<__bf16*>(b));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int pr_get_stats(const pr_handle* h, pr_stats* out) {
   if (!h || !out) return fail(PR_ERR_ARG, "NULL argument");
   *out = h->last;
@@ -591,6 +1092,12 @@ void pr_destroy(pr_handle* h) {
   if (h->d_gscr) (void)hipFree(h->d_gscr);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->ev2) (void)hipEventDestroy(h->ev2);
+  if (h->ev3) (void)hipEventDestroy(h->ev3);
+  if (h->surr) {
+    for (void* q : h->surr->owned) (void)hipFree(q);
+    delete h->surr;
+  }
   delete h;
 }
 

@@ -23,7 +23,9 @@ LIB_PATH = os.path.join(_HERE, "libpelelmex_amd.so")
 _LIB = None
 
 EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_rhs_batch", "pr_jac_batch",
-           "pr_reaction_rate_box", "pr_get_stats", "pr_num_species", "pr_destroy", "pr_last_error"]
+           "pr_reaction_rate_box", "pr_surrogate_load", "pr_surrogate_features", "pr_mlp_gemm",
+           "pr_mlp_gelu_backward", "pr_mlp_column_sum", "pr_mlp_mse", "pr_mlp_adam", "pr_mlp_to_bf16",
+           "pr_get_stats", "pr_num_species", "pr_destroy", "pr_last_error"]
 
 
 class Array4(C.Structure):
@@ -40,7 +42,10 @@ class Stats(C.Structure):
     _fields_ = [("cells", C.c_int64), ("failed", C.c_int64), ("nst", C.c_int64), ("nfe", C.c_int64),
                 ("nje", C.c_int64), ("nsetups", C.c_int64), ("nni", C.c_int64), ("netf", C.c_int64),
                 ("ncfn", C.c_int64), ("kernel_ms", C.c_double), ("lanes_per_cell", C.c_int),
-                ("blocks", C.c_int)]
+                ("blocks", C.c_int), ("sur_accepted", C.c_int64), ("sur_rejected", C.c_int64),
+                ("sur_audited", C.c_int64), ("audit_failed", C.c_int64), ("total_ms", C.c_double),
+                ("surrogate_ms", C.c_double), ("audit_ms", C.c_double), ("audit_max_dT_rel", C.c_double),
+                ("audit_max_dY", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -70,6 +75,16 @@ def lib():
                                            C.POINTER(Array4), C.POINTER(Array4), C.POINTER(Array4),
                                            C.POINTER(IArray4), C.c_void_p]
         L.pr_reaction_rate_box.restype = C.c_int
+        vp, i64, fp = C.c_void_p, C.c_int64, C.POINTER(C.c_float)
+        L.pr_surrogate_load.argtypes = [vp, C.c_int, C.POINTER(C.c_int), fp, fp, fp]
+        L.pr_surrogate_features.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
+        L.pr_mlp_gemm.argtypes = [C.c_int, C.c_int, C.c_int, vp, i64, i64, vp, i64, i64, vp, C.c_int, vp, i64, vp,
+                                  i64, vp, i64, C.c_int, vp]
+        L.pr_mlp_gelu_backward.argtypes = [i64, C.c_int, vp, i64, vp, i64, vp, i64, vp, i64, vp]
+        L.pr_mlp_column_sum.argtypes = [i64, C.c_int, vp, i64, vp, C.c_int, vp]
+        L.pr_mlp_mse.argtypes = [i64, C.c_int, vp, i64, vp, i64, vp, vp, vp]
+        L.pr_mlp_adam.argtypes = [i64, vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, vp]
+        L.pr_mlp_to_bf16.argtypes = [i64, vp, vp, vp]
         L.pr_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.pr_num_species.argtypes = [C.c_void_p]
         L.pr_destroy.argtypes = [C.c_void_p]
@@ -194,6 +209,38 @@ class Reactor:
         if rc != 0:
             raise ReactorError(f"pr_reaction_rate_box failed ({rc}): {lib().pr_last_error().decode()}")
         return rYdot
+
+    def load_surrogate(self, dims, params, xnorm, ynorm):
+        """pr_surrogate_load: dims (layer widths), params (fp32 W0, b0, W1, b1, ... flattened),
+        xnorm / ynorm ((mean, sd) stacked); host arrays."""
+        import numpy as np
+        d = np.ascontiguousarray(dims, dtype=np.int32)
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        xn = np.ascontiguousarray(xnorm, dtype=np.float32)
+        yn = np.ascontiguousarray(ynorm, dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        rc = lib().pr_surrogate_load(self._h, d.size - 1, d.ctypes.data_as(C.POINTER(C.c_int)), p.ctypes.data_as(fp),
+                                     xn.ctypes.data_as(fp), yn.ctypes.data_as(fp))
+        if rc != 0:
+            raise ReactorError(f"pr_surrogate_load failed ({rc}): {lib().pr_last_error().decode()}")
+
+    def surrogate_features(self, rY, T, dt, normalised=False, stream=None):
+        """Network inputs for a batch (device tensors rY [N, nc], T [nc], dt [nc]): raw fp32
+        [nc, N + 2], or normalised bf16 with a loaded network."""
+        import torch
+        nc = T.numel()
+        st = stream if stream is not None else torch.cuda.current_stream()
+        if normalised:
+            out = torch.empty((nc, self.nsp + 2), dtype=torch.bfloat16, device=T.device)
+            args = (None, out.data_ptr())
+        else:
+            out = torch.empty((nc, self.nsp + 2), dtype=torch.float32, device=T.device)
+            args = (out.data_ptr(), None)
+        rc = lib().pr_surrogate_features(self._h, nc, rY.data_ptr(), T.data_ptr(), dt.data_ptr(), *args,
+                                         C.c_void_p(st.cuda_stream))
+        if rc != 0:
+            raise ReactorError(f"pr_surrogate_features failed ({rc}): {lib().pr_last_error().decode()}")
+        return out
 
     def stats(self):
         s = Stats()

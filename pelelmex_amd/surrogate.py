@@ -1,0 +1,263 @@
+"""Learned dt_react map in front of the BDF: dataset, training and loading.
+
+The network maps a cell's state (log10(Y_k + 1e-20), T / 2000, log10(dt)) to its chemical
+increments (dY_k, dT) over dt. Training pairs come from the library's GPU BDF (pr_react_box) on
+states drawn like the flame-box workload, without external sources (the map learns chemistry;
+sources are added exactly in the trust kernel). Forward, backward and Adam run on the library's
+own kernels (pr_mlp_*: BF16 matrix-core products with FP32 accumulation); torch only provides
+device memory. The trained network is stored as a .npz of float32 arrays and handed to a
+Reactor with pr_surrogate_load, after which pr_react_box accepts the cells that pass the FP64
+trust check and integrates the others with the BDF.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import time
+
+import numpy as np
+
+from . import workloads
+from .reactor import Reactor, ReactorError, lib
+
+HIDDEN = (256, 256, 256, 256)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise ReactorError(f"{what} failed ({rc}): {lib().pr_last_error().decode()}")
+
+
+def gemm(M, N, K, A, sam, sak, B, sbk, sbn, bias=None, gelu=False, Cf=None, ldc=0, Cb=None, ldcb=0, Z=None, ldz=0,
+         accumulate=False):
+    """pr_mlp_gemm on device tensors (A, B bf16)."""
+    _chk(lib().pr_mlp_gemm(M, N, K, _ptr(A), sam, sak, _ptr(B), sbk, sbn, _ptr(bias), int(gelu), _ptr(Cf), ldc,
+                           _ptr(Cb), ldcb, _ptr(Z), ldz, int(accumulate), _stream()), "pr_mlp_gemm")
+
+
+class MLP:
+    """Dense GELU network with fp32 master parameters and bf16 copies on the device."""
+
+    def __init__(self, dims, device="cuda", seed=0):
+        import torch
+        self.dims = list(dims)
+        self.device = device
+        g = torch.Generator().manual_seed(seed)
+        self.W, self.b = [], []
+        for i, o in zip(self.dims[:-1], self.dims[1:]):
+            std = math.sqrt(2.0 / i) if len(self.W) < len(self.dims) - 2 else math.sqrt(1.0 / i)
+            self.W.append((torch.randn(i, o, generator=g, dtype=torch.float32) * std).to(device))
+            self.b.append(torch.zeros(o, dtype=torch.float32, device=device))
+        self._init_state()
+
+    def _init_state(self):
+        import torch
+        self.Wb = [w.to(torch.bfloat16) for w in self.W]
+        self.gW = [torch.zeros_like(w) for w in self.W]
+        self.gb = [torch.zeros_like(b) for b in self.b]
+        self.m = [torch.zeros_like(p) for p in self.W + self.b]
+        self.v = [torch.zeros_like(p) for p in self.W + self.b]
+        self.step = 0
+
+    @property
+    def nlayers(self):
+        return len(self.W)
+
+    def refresh_bf16(self):
+        for w, wb in zip(self.W, self.Wb):
+            _chk(lib().pr_mlp_to_bf16(w.numel(), _ptr(w), _ptr(wb), _stream()), "pr_mlp_to_bf16")
+
+    def forward(self, X, keep=False):
+        """X: bf16 [B, dims[0]] -> fp32 [B, dims[-1]]; keep=True stores what backward needs."""
+        import torch
+        B = X.shape[0]
+        acts, pre = [X], []
+        a = X
+        for l in range(self.nlayers):
+            i, o = self.dims[l], self.dims[l + 1]
+            last = l == self.nlayers - 1
+            if last:
+                out = torch.empty((B, o), dtype=torch.float32, device=X.device)
+                gemm(B, o, i, a, i, 1, self.Wb[l], o, 1, bias=self.b[l], Cf=out, ldc=o)
+                a = out
+            else:
+                z = torch.empty((B, o), dtype=torch.float32, device=X.device) if keep else None
+                nxt = torch.empty((B, o), dtype=torch.bfloat16, device=X.device)
+                gemm(B, o, i, a, i, 1, self.Wb[l], o, 1, bias=self.b[l], gelu=True, Cb=nxt, ldcb=o, Z=z, ldz=o)
+                pre.append(z)
+                acts.append(nxt)
+                a = nxt
+        if keep:
+            self._acts, self._pre = acts, pre
+        return a
+
+    def backward(self, dY):
+        """dY: fp32 [B, dims[-1]] gradient of the loss wrt the output; fills gW, gb."""
+        import torch
+        B = dY.shape[0]
+        dZ = dY.contiguous()
+        dZb = torch.empty(dZ.shape, dtype=torch.bfloat16, device=dY.device)
+        _chk(lib().pr_mlp_to_bf16(dZ.numel(), _ptr(dZ), _ptr(dZb), _stream()), "pr_mlp_to_bf16")
+        for l in range(self.nlayers - 1, -1, -1):
+            i, o = self.dims[l], self.dims[l + 1]
+            a = self._acts[l]
+            # gW = A^T dZ  (M = i, N = o, K = B)
+            gemm(i, o, B, a, 1, i, dZb, o, 1, Cf=self.gW[l], ldc=o)
+            _chk(lib().pr_mlp_column_sum(B, o, _ptr(dZ), o, _ptr(self.gb[l]), 0, _stream()), "pr_mlp_column_sum")
+            if l == 0:
+                break
+            # dA = dZ W^T  (M = B, N = i, K = o), then dZ_prev = dA * gelu'(Z_prev)
+            dA = torch.empty((B, i), dtype=torch.float32, device=dY.device)
+            gemm(B, i, o, dZb, o, 1, self.Wb[l], 1, o, Cf=dA, ldc=i)
+            dZ = torch.empty((B, i), dtype=torch.float32, device=dY.device)
+            dZb = torch.empty((B, i), dtype=torch.bfloat16, device=dY.device)
+            z = self._pre[l - 1]
+            _chk(lib().pr_mlp_gelu_backward(B, i, _ptr(dA), i, _ptr(z), i, _ptr(dZb), i, _ptr(dZ), i, _stream()),
+                 "pr_mlp_gelu_backward")
+
+    def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+        self.step += 1
+        params = self.W + self.b
+        grads = self.gW + self.gb
+        bf = self.Wb + [None] * len(self.b)
+        for p, g, m, v, pb in zip(params, grads, self.m, self.v, bf):
+            _chk(lib().pr_mlp_adam(p.numel(), _ptr(p), _ptr(g), _ptr(m), _ptr(v), _ptr(pb), lr, b1, b2, eps,
+                                   self.step, _stream()), "pr_mlp_adam")
+
+    def mse(self, Y, T):
+        """Loss and its gradient (fp32) for outputs Y against targets T."""
+        import torch
+        dY = torch.empty_like(Y)
+        loss = torch.zeros(1, dtype=torch.float32, device=Y.device)
+        _chk(lib().pr_mlp_mse(Y.shape[0], Y.shape[1], _ptr(Y), Y.shape[1], _ptr(T), T.shape[1], _ptr(dY), _ptr(loss),
+                              _stream()), "pr_mlp_mse")
+        return loss, dY
+
+    def flat_params(self):
+        out = []
+        for w, b in zip(self.W, self.b):
+            out += [w.detach().float().cpu().numpy().ravel(), b.detach().float().cpu().numpy().ravel()]
+        return np.concatenate(out).astype(np.float32)
+
+    def set_flat_params(self, flat):
+        import torch
+        off = 0
+        for l in range(self.nlayers):
+            i, o = self.dims[l], self.dims[l + 1]
+            self.W[l].copy_(torch.as_tensor(flat[off:off + i * o].reshape(i, o)))
+            off += i * o
+            self.b[l].copy_(torch.as_tensor(flat[off:off + o]))
+            off += o
+        self.refresh_bf16()
+
+
+def generate_pairs(reactor, seeds, dts, nx=64, device="cuda", log=print):
+    """(state, result) pairs from the GPU BDF on flame-box states without external sources.
+    Returns device tensors rY0 [N, P], T0 [P], dt [P], rY1 [N, P], T1 [P]."""
+    import torch
+    from .reactor import box_tensors
+    out = {k: [] for k in ("rY0", "T0", "dt", "rY1", "T1")}
+    for seed, dt in zip(seeds, dts):
+        box = workloads.ch4_air_flame_box(nx=nx, ny=nx, nz=nx, seed=seed, dt=dt, src_frac=0.0)
+        t = box_tensors(box, device)
+        rY0, T0 = t["rY"].reshape(reactor.nsp, -1).clone(), t["T"].reshape(-1).clone()
+        t0 = time.perf_counter()
+        nfail, _, _ = reactor.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], None, dt, 0.0)
+        torch.cuda.synchronize()
+        log(f"  pairs: seed {seed} dt {dt:.2e} cells {T0.numel()} BDF {time.perf_counter() - t0:.2f} s, failed {nfail}")
+        ok = torch.ones(T0.numel(), dtype=torch.bool, device=device) if nfail == 0 else None
+        if ok is None:
+            raise ReactorError(f"{nfail} cells failed while generating training pairs")
+        out["rY0"].append(rY0)
+        out["T0"].append(T0)
+        out["dt"].append(torch.full_like(T0, dt))
+        out["rY1"].append(t["rY"].reshape(reactor.nsp, -1).clone())
+        out["T1"].append(t["T"].reshape(-1).clone())
+    return {"rY0": torch.cat(out["rY0"], 1), "T0": torch.cat(out["T0"]), "dt": torch.cat(out["dt"]),
+            "rY1": torch.cat(out["rY1"], 1), "T1": torch.cat(out["T1"])}
+
+
+def targets(pairs):
+    """Chemical increments [dY_k (N), dT] per pair, fp64 [P, N + 1]."""
+    import torch
+    rho0 = pairs["rY0"].sum(0)
+    rho1 = pairs["rY1"].sum(0)
+    dY = pairs["rY1"] / rho1 - pairs["rY0"] / rho0
+    dT = pairs["T1"] - pairs["T0"]
+    return torch.cat([dY, dT[None, :]], 0).T.contiguous()
+
+
+def train(reactor, pairs, hidden=HIDDEN, epochs=20, batch=4096, lr=1e-3, seed=0, log=print, time_budget=None):
+    """Trains an MLP on the pairs with the library's kernels; returns (mlp, xnorm, ynorm, history)."""
+    import torch
+    dev = pairs["T0"].device
+    N = reactor.nsp
+    raw = reactor.surrogate_features(pairs["rY0"].contiguous(), pairs["T0"], pairs["dt"])
+    xmu = raw.double().mean(0)
+    xsd = raw.double().std(0).clamp_min(1e-3)
+    tgt = targets(pairs)
+    ymu = tgt.mean(0)
+    ysd = tgt.std(0).clamp_min(1e-12)
+    xnorm = torch.stack([xmu, xsd]).float().cpu().numpy()
+    ynorm = torch.stack([ymu, ysd]).float().cpu().numpy()
+    X = ((raw - xmu.float()) / xsd.float()).to(torch.bfloat16).contiguous()
+    Y = ((tgt - ymu) / ysd).float().contiguous()
+    P = X.shape[0]
+    dims = [N + 2, *hidden, N + 1]
+    mlp = MLP(dims, device=dev, seed=seed)
+    g = torch.Generator(device=dev).manual_seed(seed)
+    hist = []
+    t0 = time.perf_counter()
+    steps_per_epoch = P // batch
+    for ep in range(epochs):
+        perm = torch.randperm(P, device=dev, generator=g)
+        tot = torch.zeros(1, dtype=torch.float32, device=dev)
+        for s in range(steps_per_epoch):
+            idx = perm[s * batch:(s + 1) * batch]
+            xb, yb = X[idx].contiguous(), Y[idx].contiguous()
+            out = mlp.forward(xb, keep=True)
+            loss, dY = mlp.mse(out, yb)
+            mlp.backward(dY)
+            lr_t = lr * 0.5 * (1.0 + math.cos(math.pi * (ep * steps_per_epoch + s) / (epochs * steps_per_epoch)))
+            mlp.adam(lr=lr_t)
+            tot += loss
+        torch.cuda.synchronize()
+        hist.append(float(tot.item()) / max(steps_per_epoch, 1))
+        log(f"  epoch {ep + 1}/{epochs}: mean loss {hist[-1]:.4e} ({time.perf_counter() - t0:.1f} s)")
+        if time_budget and time.perf_counter() - t0 > time_budget:
+            break
+    return mlp, xnorm, ynorm, hist
+
+
+def save(path, mlp, xnorm, ynorm, **meta):
+    np.savez(path, dims=np.asarray(mlp.dims, dtype=np.int32), params=mlp.flat_params(),
+             xnorm=np.asarray(xnorm, np.float32), ynorm=np.asarray(ynorm, np.float32),
+             **{k: np.asarray(v) for k, v in meta.items()})
+
+
+def load(path):
+    """Dict with dims, params, xnorm, ynorm and the stored metadata (numpy arrays)."""
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def attach(reactor, model):
+    """Loads a trained network (dict from load(), or an (mlp, xnorm, ynorm) triple) into a Reactor."""
+    if isinstance(model, dict):
+        reactor.load_surrogate(model["dims"], model["params"], model["xnorm"], model["ynorm"])
+    else:
+        mlp, xnorm, ynorm = model
+        reactor.load_surrogate(mlp.dims, mlp.flat_params(), xnorm, ynorm)
+
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "surrogate_drm19.npz")
