@@ -1,4 +1,4 @@
-"""Benchmark: chem cells/s per dt_react of the plain GPU BDF reactor on one MI355X.
+"""Benchmark: chem cells/s per dt_react on one MI355X (learned map + BDF fallback, and plain GPU BDF).
 
 Workload (one "step" = one ReactorBase::react call over the whole box, inputs resident in HBM):
 drm19 CH4/air flame-like box, 64^3 = 262144 cells of uneven stiffness (30 % cold unburnt at
@@ -6,6 +6,10 @@ drm19 CH4/air flame-like box, 64^3 = 262144 cells of uneven stiffness (30 % cold
 of |rhoY|/dt and |rhoH|/dt, dt_react = 5e-6 s, 1 atm, rtol = atol = 1e-10, cvode.max_order = 4
 (the value the reference's production inputs use), FP64 throughout. The box is restored from a
 pristine device copy before each react call, so every step integrates the same states.
+
+With a trained network (pelelmex_amd/surrogate_drm19.npz) the metric value is the learned map +
+FP64 trust check + BDF fallback path's throughput; the plain GPU BDF is timed on the same box in
+the same run (bdf_value) and the surrogate result is compared with it on every cell.
 
 Prints one JSON line. `roofline` is the integration kernel's algorithmic FP64 rate
 (pelelmex_amd.roofline counts, kernel time from HIP events on the launch stream) against the
@@ -88,6 +92,8 @@ def main():
     ap.add_argument("--max-order", type=int, default=4)
     ap.add_argument("--cpu-sample", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true", help="measure the plain GPU BDF only")
+    ap.add_argument("--surrogate", default=None, help="trained network (.npz); default: the packaged one")
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device copy of this size (calibrates byte counters in a profile)")
@@ -98,7 +104,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from pelelmex_amd import roofline, workloads
+    from pelelmex_amd import roofline, surrogate, workloads
     from pelelmex_amd.reactor import Reactor, box_tensors
 
     box = workloads.ch4_air_flame_box(nx=args.nx, ny=args.nx, nz=args.nx)
@@ -109,16 +115,27 @@ def main():
     keys = {"ode.rtol": 1e-10, "ode.atol": 1e-10, "cvode.max_order": args.max_order,
             "cvode.max_substeps": 10000, "cvode.solve_type": "denseAJ_direct",
             "hipbdf.lanes_per_cell": args.lanes}
-    r = Reactor.create("ReactorCvode", "drm19", keys)
-    r.init(2, 1)
 
-    def step():
-        for k in pristine:
-            t[k].copy_(pristine[k])
-        nfail, _, _ = r.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], t["mask"], box["dt"], 0.0)
-        if nfail:
-            raise RuntimeError(f"{nfail} cells failed to integrate")
-        return r.stats()
+    def measure(r):
+        """W untimed + K timed react calls on the pristine box; (ms per call, last stats, state)."""
+        def step():
+            for k in pristine:
+                t[k].copy_(pristine[k])
+            nfail, _, _ = r.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], t["mask"], box["dt"],
+                                  0.0)
+            if nfail:
+                raise RuntimeError(f"{nfail} cells failed to integrate")
+            return r.stats()
+        for _ in range(args.warmup):
+            step()
+        ref_T = t["T"].clone() if args.warmup > 0 else None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        kstats = [step() for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        det = bool(torch.equal(ref_T, t["T"])) if ref_T is not None else None
+        return ms, kstats, det, {k: t[k].clone() for k in ("rY", "T", "FC")}
 
     if args.copy_probe_mb:
         a = torch.empty(args.copy_probe_mb * (1 << 20) // 8, dtype=torch.float64, device=dev).fill_(1.0)
@@ -127,19 +144,12 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    for _ in range(args.warmup):
-        step()
-    ref_T = t["T"].clone()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    kstats = []
-    for _ in range(args.steps):
-        kstats.append(step())
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    deterministic = bool(torch.equal(ref_T, t["T"])) if args.warmup > 0 else None
-    value = ncells / (ms_per_step / 1e3)
+
+    r = Reactor.create("ReactorCvode", "drm19", keys)
+    r.init(2, 1)
+    ms_bdf, kstats, deterministic, bdf_state = measure(r)
+    r.close()
+    bdf_value = ncells / (ms_bdf / 1e3)
 
     kernel_ms = float(np.mean([s["kernel_ms"] for s in kstats]))
     st = kstats[-1]
@@ -149,6 +159,37 @@ def main():
     workload_key = f"drm19 {args.nx}^3 lanes {st['lanes_per_cell']} max_order {args.max_order}"
     traffic, traffic_src = profiled_traffic(workload_key)
 
+    # learned map + BDF fallback on the same box
+    sur = None
+    wpath = args.surrogate or surrogate.WEIGHTS
+    if not args.no_surrogate and os.path.exists(wpath):
+        model = surrogate.load(wpath)
+        skeys = dict(keys)
+        if "trust_rtol" in model:
+            skeys["surrogate.trust_rtol"] = float(model["trust_rtol"])
+            skeys["surrogate.trust_atol"] = float(model["trust_atol"])
+        rs = Reactor.create("ReactorCvode", "drm19", skeys)
+        rs.init(2, 1)
+        surrogate.attach(rs, model)
+        ms_sur, sstats, _, sur_state = measure(rs)
+        rs.close()
+        s2 = sstats[-1]
+        N = bdf_state["rY"].shape[0]
+        dT = ((sur_state["T"] - bdf_state["T"]).abs() / bdf_state["T"]).max().item()
+        Ys = sur_state["rY"].reshape(N, -1) / sur_state["rY"].reshape(N, -1).sum(0)
+        Yb = bdf_state["rY"].reshape(N, -1) / bdf_state["rY"].reshape(N, -1).sum(0)
+        sur = {"value": ncells / (ms_sur / 1e3), "ms_per_step": ms_sur,
+               "accepted": s2["sur_accepted"] / ncells, "rejected_to_bdf": s2["sur_rejected"] / ncells,
+               "trust_rtol": skeys.get("surrogate.trust_rtol"), "trust_atol": skeys.get("surrogate.trust_atol"),
+               "network_and_trust_ms": s2["surrogate_ms"], "fallback_bdf_ms": s2["kernel_ms"],
+               "audit_ms": s2["audit_ms"], "audited": s2["sur_audited"], "audit_max_dT_rel": s2["audit_max_dT_rel"],
+               "audit_max_dY": s2["audit_max_dY"],
+               "max_dT_rel_vs_bdf_all_cells": dT,
+               "max_dY_vs_bdf_all_cells": (Ys - Yb).abs().max().item(),
+               "speedup_vs_bdf": bdf_value and (ncells / (ms_sur / 1e3)) / bdf_value,
+               "weights": os.path.relpath(wpath, ROOT)}
+
+    value = sur["value"] if sur else bdf_value
     line = {
         "metric": METRIC,
         "value": value,
@@ -156,23 +197,27 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": sur["ms_per_step"] if sur else ms_bdf,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"plain GPU BDF, drm19 CH4/air flame-like box {args.nx}^3 = {ncells} cells, "
-                               "dt_react 5e-6 s, 1 atm, 5% Gaussian sources",
+        "config": {"workload": f"drm19 CH4/air flame-like box {args.nx}^3 = {ncells} cells, dt_react 5e-6 s, "
+                               "1 atm, 5% Gaussian sources",
                    "mechanism": "drm19", "cells": ncells, "dt_react": box["dt"], "rtol": 1e-10, "atol": 1e-10,
                    "max_order": args.max_order, "lanes_per_cell": st["lanes_per_cell"], "seed": 2002,
-                   "path": "plain BDF (surrogate not in this build)"},
+                   "path": "learned map (BF16 MFMA) + FP64 trust check + GPU BDF fallback" if sur
+                   else "plain GPU BDF"},
+        "bdf_value": bdf_value,
+        "bdf_ms_per_step": ms_bdf,
+        "surrogate": sur,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": roofline.MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / roofline.MI355X_FP64_TFLOPS, "traffic": traffic,
                      "traffic_source": (f"profiles/{traffic_src} (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a "
                                         "1 GiB device copy)") if traffic else "no profile of these kernel sources",
-                     "kernel": "react_kernel<21,84,%d>" % st["lanes_per_cell"], "kernel_ms": kernel_ms,
-                     "flops_per_launch": F,
+                     "kernel": "react_kernel<21,84,%d> (plain BDF over the whole box)" % st["lanes_per_cell"],
+                     "kernel_ms": kernel_ms, "flops_per_launch": F,
                      "hbm_algorithmic_bytes": counts["bytes_cell"] * ncells,
                      "hbm_algorithmic_GBs": counts["bytes_cell"] * ncells / (kernel_ms / 1e3) / 1e9},
         "integrator": {"steps_per_cell": st["nst"] / st["cells"], "rhs_per_cell": st["nfe"] / st["cells"],

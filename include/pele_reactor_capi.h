@@ -106,8 +106,11 @@ int pr_reaction_rate_box(pr_handle* h, const int lo[3], const int hi[3], const p
 /* Learned dt_react map. Network: nlayers dense layers, dims[0] = NUM_SPECIES + 2 inputs
    (log10(Y_k + 1e-20), T / 2000, log10(dt)), dims[nlayers] = NUM_SPECIES + 1 outputs (dY_k, dT),
    GELU between layers. params (host, fp32): for each layer W [dims[l] x dims[l+1]] row-major then
-   b [dims[l+1]]. xnorm: mean then standard deviation of the inputs (2 x dims[0]); ynorm: same
-   for the outputs (2 x dims[nlayers]). Once loaded (and surrogate.enable != 0) pr_react_box
+   b [dims[l+1]]. xnorm: mean then standard deviation of the inputs (2 x dims[0]); ynorm: mean,
+   standard deviation and rate-scale floor eps of the outputs (3 x dims[nlayers]): with
+   u_i = out_i sd_i + mean_i the increment is d_i = u_i (dt |f_i(y*)| / rho* + eps_i) for species
+   and u_T (dt |f_T(y*)| + eps_T) for T, f the source-free reactor right-hand side at the
+   source-advanced state y* = y0 + dt src (or d_i = u_i when eps_i = 0). Once loaded (and surrogate.enable != 0) pr_react_box
    evaluates it in BF16 on the matrix cores, accepts the cells that pass the FP64 trust check
    (element balance, backward-Euler residual with the analytic rates at surrogate.trust_rtol /
    surrogate.trust_atol) and integrates the others with the BDF; surrogate.audit_frac of the

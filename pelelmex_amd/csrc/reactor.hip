@@ -58,6 +58,7 @@ using Occ = hipError_t (*)(int* nb, int block, size_t shmem);
 using RateLaunch = hipError_t (*)(const prk::RateParams&, int blocks, int block, size_t shmem, hipStream_t);
 using TrustLaunch = hipError_t (*)(const prk::TrustParams&, int blocks, int block, size_t shmem, hipStream_t);
 using FeatLaunch = hipError_t (*)(const prk::FeatParams&, hipStream_t);
+using PrepLaunch = hipError_t (*)(const prk::PrepParams&, int blocks, int block, size_t shmem, hipStream_t);
 
 template <int N, int R, int G>
 hipError_t launch_t(const prk::KParams& P, int blocks, int block, size_t shmem, hipStream_t st) {
@@ -90,6 +91,12 @@ hipError_t trust_t(const prk::TrustParams& P, int blocks, int block, size_t shme
   return hipGetLastError();
 }
 
+template <int N, int R, int G>
+hipError_t prep_t(const prk::PrepParams& P, int blocks, int block, size_t shmem, hipStream_t st) {
+  hipLaunchKernelGGL((prk::prep_kernel<N, R, G>), dim3(blocks), dim3(block), shmem, st, P);
+  return hipGetLastError();
+}
+
 template <int N>
 hipError_t feat_t(const prk::FeatParams& P, hipStream_t st) {
   if (P.ncells <= 0) return hipSuccess;
@@ -119,11 +126,12 @@ struct Variant {
   RateLaunch rate;
   TrustLaunch trust;
   FeatLaunch feat;
+  PrepLaunch prep;
 };
 
 #define PR_VARIANT(NAME, NS, NR, G) \
   {NAME, G, &launch_t<NS, NR, G>, &eval_t<NS, NR, G>, &occ_t<NS, NR, G>, scratch_bytes<NS, NR, G>(), \
-   prk::Cell<NS, NR, G>::gslot_doubles * 8, &rate_t<NS, NR, G>, &trust_t<NS, NR, G>, &feat_t<NS>}
+   prk::Cell<NS, NR, G>::gslot_doubles * 8, &rate_t<NS, NR, G>, &trust_t<NS, NR, G>, &feat_t<NS>, &prep_t<NS, NR, G>}
 
 const Variant kVariants[] = {
     PR_VARIANT("LiDryer", PR_LIDRYER_NSP, PR_LIDRYER_NR, 16),
@@ -322,12 +330,14 @@ struct Surrogate {
   std::vector<int> dims;            // layer widths, dims[0] = N + 2, back = N + 1
   std::vector<__bf16*> W;           // [dims[l] x dims[l+1]] bf16
   std::vector<float*> b;            // [dims[l+1]]
-  float *xmu = nullptr, *xsd = nullptr, *ymu = nullptr, *ysd = nullptr;
+  float *xmu = nullptr, *xsd = nullptr, *ymu = nullptr, *ysd = nullptr, *yeps = nullptr;
   double *Mel = nullptr, *Minv = nullptr;
   int E = 0;
   long cap = 0;                     // cells the buffers hold
   __bf16 *X = nullptr, *H0 = nullptr, *H1 = nullptr;
   float* Yout = nullptr;
+  double* Tstar = nullptr;
+  double* scale = nullptr;
   int* rej = nullptr;
   unsigned int* cnt = nullptr;      // [0] rejected [1] audited [2] accepted [3] fallback queue [4] audit queue
   unsigned int aud_cap = 0;
@@ -489,7 +499,7 @@ static int ensure_surr_buffers(pr_handle* h, long ncells) {
   Surrogate* S = h->surr;
   if (ncells <= S->cap) return 0;
   const int N = h->mech->nsp;
-  for (void* q : {(void*)S->X, (void*)S->H0, (void*)S->H1, (void*)S->Yout, (void*)S->rej, (void*)S->aud_in,
+  for (void* q : {(void*)S->X, (void*)S->H0, (void*)S->H1, (void*)S->Yout, (void*)S->Tstar, (void*)S->scale, (void*)S->rej, (void*)S->aud_in,
                   (void*)S->aud_out, (void*)S->aud_box, (void*)S->aud_err, (void*)S->aud_cell})
     if (q) {
       (void)hipFree(q);
@@ -512,6 +522,8 @@ static int ensure_surr_buffers(pr_handle* h, long ncells) {
       (rc = surr_alloc(S, (void**)&S->H1, sizeof(__bf16) * ncells * hid)) ||
       (rc = surr_alloc(S, (void**)&S->Yout, sizeof(float) * ncells * (N + 1))) ||
       (rc = surr_alloc(S, (void**)&S->rej, sizeof(int) * ncells)) ||
+      (rc = surr_alloc(S, (void**)&S->Tstar, sizeof(double) * ncells)) ||
+      (rc = surr_alloc(S, (void**)&S->scale, sizeof(double) * ncells * (N + 1))) ||
       (rc = surr_alloc(S, (void**)&S->aud_in, sizeof(double) * ac * (2 * N + 3))) ||
       (rc = surr_alloc(S, (void**)&S->aud_out, sizeof(double) * ac * (N + 1))) ||
       (rc = surr_alloc(S, (void**)&S->aud_box, sizeof(double) * ac * (2 * N + 4))) ||
@@ -624,7 +636,14 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   if (S) {
     int rc = ensure_surr_buffers(h, ncells);
     if (rc) return rc;
-    prk::FeatParams F{};
+    const int G = h->var->G;
+    const int block = pick_block(h);
+    const size_t shmem = (size_t)h->blob_bytes + (size_t)(block / G) * h->var->scratch;
+    int dev_cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    const long want = (ncells + (block / G) - 1) / (block / G);
+    const int gblocks = (int)(want < 8L * dev_cus ? want : 8L * dev_cus);
+    prk::PrepParams F{};
     F.lo0 = lo[0];
     F.lo1 = lo[1];
     F.lo2 = lo[2];
@@ -632,12 +651,23 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
     F.ny = (int)ny;
     F.ncells = ncells;
     F.rY = P.rY;
+    F.rYsrc = P.rYsrc;
     F.T = P.T;
+    F.rH = P.rH;
+    F.rHsrc = P.rHsrc;
+    F.mask = P.mask;
+    F.has_mask = P.has_mask;
     F.dt = dt;
     F.xmu = S->xmu;
     F.xsd = S->xsd;
     F.X = S->X;
-    HIPCHK(h->var->feat(F, st));
+    F.Tstar = S->Tstar;
+    F.scale = S->scale;
+    F.blob = h->d_blob;
+    F.blob_bytes = h->blob_bytes;
+    F.neff = h->mech->neff;
+    F.ninc = h->mech->ninc;
+    HIPCHK(h->var->prep(F, gblocks, block, shmem, st));
     if ((rc = mlp_forward(h, ncells, st))) return rc;
     HIPCHK(hipMemsetAsync(S->cnt, 0, 8 * sizeof(unsigned int), st));
     prk::TrustParams Q{};
@@ -660,6 +690,7 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
     Q.Yout = S->Yout;
     Q.ymu = S->ymu;
     Q.ysd = S->ysd;
+    Q.yeps = S->yeps;
     Q.Mel = S->Mel;
     Q.Minv = S->Minv;
     Q.E = S->E;
@@ -679,14 +710,9 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
     Q.aud_in = S->aud_in;
     Q.aud_out = S->aud_out;
     Q.aud_cell = S->aud_cell;
-    const int G = h->var->G;
-    const int block = pick_block(h);
-    const size_t shmem = (size_t)h->blob_bytes + (size_t)(block / G) * h->var->scratch;
-    int dev_cus = 0;
-    HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, h->device));
-    const long want = (ncells + (block / G) - 1) / (block / G);
-    const long cap = 8L * dev_cus;
-    HIPCHK(h->var->trust(Q, (int)(want < cap ? want : cap), block, shmem, st));
+    Q.Tstar = S->Tstar;
+    Q.scale = S->scale;
+    HIPCHK(h->var->trust(Q, gblocks, block, shmem, st));
     P.cell_list = S->rej;
     P.ncells_dev = &S->cnt[0];
   }
@@ -891,8 +917,10 @@ int pr_surrogate_load(pr_handle* h, int nlayers, const int* dims, const float* p
     if (dims[l] < 1 || dims[l] > 8192) return fail(PR_ERR_ARG, "layer width out of range");
   for (int l = 0; l < 2 * dims[0]; ++l)
     if (!std::isfinite(xnorm[l]) || (l >= dims[0] && !(xnorm[l] > 0))) return fail(PR_ERR_ARG, "bad xnorm");
-  for (int l = 0; l < 2 * dims[nlayers]; ++l)
-    if (!std::isfinite(ynorm[l]) || (l >= dims[nlayers] && !(ynorm[l] > 0))) return fail(PR_ERR_ARG, "bad ynorm");
+  for (int l = 0; l < 3 * dims[nlayers]; ++l)
+    if (!std::isfinite(ynorm[l]) || (l >= dims[nlayers] && l < 2 * dims[nlayers] && !(ynorm[l] > 0)) ||
+        (l >= 2 * dims[nlayers] && ynorm[l] < 0))
+      return fail(PR_ERR_ARG, "bad ynorm");
   // element matrix M_ek = a_ek / W_k and (M M^T)^-1 by Gauss-Jordan
   const int E = h->mech->nelem;
   if (E < 1 || E > prk::SURR_MAXE) return fail(PR_ERR_UNSUPPORTED, "too many elements for the trust check");
@@ -960,15 +988,16 @@ int pr_surrogate_load(pr_handle* h, int nlayers, const int* dims, const float* p
     off += nw + nb;
   }
   const int nin = dims[0], nout = dims[nlayers];
-  if (surr_alloc(S, (void**)&S->xmu, 2 * nin * sizeof(float)) || surr_alloc(S, (void**)&S->ymu, 2 * nout * sizeof(float)) ||
+  if (surr_alloc(S, (void**)&S->xmu, 2 * nin * sizeof(float)) || surr_alloc(S, (void**)&S->ymu, 3 * nout * sizeof(float)) ||
       surr_alloc(S, (void**)&S->Mel, Mel.size() * sizeof(double)) ||
       surr_alloc(S, (void**)&S->Minv, Minv.size() * sizeof(double)) ||
       surr_alloc(S, (void**)&S->cnt, 8 * sizeof(unsigned int)))
     return cleanup(PR_ERR_HIP, "device allocation failed");
   S->xsd = S->xmu + nin;
   S->ysd = S->ymu + nout;
+  S->yeps = S->ymu + 2 * nout;
   if (hipMemcpy(S->xmu, xnorm, 2 * nin * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(S->ymu, ynorm, 2 * nout * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(S->ymu, ynorm, 3 * nout * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(S->Mel, Mel.data(), Mel.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(S->Minv, Minv.data(), Minv.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(PR_ERR_HIP, "upload failed");

@@ -4,13 +4,15 @@
 // asks for, while every cell it cannot vouch for is integrated by the BDF kernel.
 //
 // Per cell, the network maps x = [log10(Y_k + 1e-20) (N), T / 2000, log10(dt)] (z-normalised)
-// to the chemical increments [dY_k (N), dT] over dt (z-normalised). The trust kernel then
+// to the chemical increments [dY_k (N), dT] over dt relative to the explicit estimate from the
+// analytic rates at the source-advanced state, u_i = d_i / (dt |f_i(y*)| (/ rho*) + eps_i), then
+// z-normalised: frozen cells give d = 0 and near-equilibrium cells O(1) targets. The trust kernel then
 //  * removes the increments' element imbalance by the least-squares projection
 //    dY <- dY - M^T (M M^T)^-1 M dY, M_ek = a_ek / W_k (elements per gram), and rejects a
 //    cell whose raw imbalance exceeds elem_tol of its gross change,
-//  * builds the state exactly as the BDF would for the energy and source terms:
-//    rhoY1 = rho0 (Y0 + dY) + dt src, rhoh1 = rhoh0 + dt src_h, T1 = HY2T(rhoh1 / rho1, Y1)
-//    started at T0 + dT,
+//  * applies the sources first and exactly (prep kernel): y* = rhoY0 + dt src, rhoh1 =
+//    rhoh0 + dt src_h, T* = HY2T(rhoh1 / rho*, Y*); the network sees (Y*, T*, dt), and the cell
+//    becomes rhoY1 = rho* (Y* + dY), T1 = HY2T(rhoh1 / rho1, Y1) started at T* + dT,
 //  * evaluates the backward-Euler residual r = y1 - y0 - dt f(dt, y1) with the analytic rates
 //    (f is the reactor right-hand side the BDF integrates) and accepts the cell when
 //    ||r||_wrms <= 1 with weights 1 / (trust_rtol |y1_i| + trust_atol), rhoY1 >= -trust_atol
@@ -41,6 +43,15 @@ struct FeatParams {
   __bf16* X;         // [ncells x (N + 2)] normalised bf16 features, nullable
 };
 
+// feature f of a cell: log10(max(Y_f, 0) + 1e-20) for f < N, T / 2000, log10(dt)
+__device__ __forceinline__ double feature_value(int f, int N, double Y, double T, double dt) {
+  if (f < N) return log10(fmax(Y, 0.0) + 1e-20);
+  if (f == N) return T / 2000.0;
+  return log10(dt);
+}
+
+__device__ __forceinline__ __bf16 feature_norm(double x, float mu, float sd) { return (__bf16)(((float)x - mu) / sd); }
+
 template <int N>
 __global__ void featurize_kernel(FeatParams P) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -70,12 +81,101 @@ __global__ void featurize_kernel(FeatParams P) {
   const int F = N + 2;
 #pragma unroll
   for (int f = 0; f < F; ++f) {
-    double x;
-    if (f < N) x = log10(fmax(ry[f] / rho, 0.0) + 1e-20);
-    else if (f == N) x = P.T.p[offT] / 2000.0;
-    else x = log10(dt);
+    const double x = feature_value(f, N, f < N ? ry[f] / rho : 0.0, P.T.p[offT], dt);
     if (P.raw) P.raw[c * F + f] = (float)x;
-    if (P.X) P.X[c * F + f] = (__bf16)(((float)x - P.xmu[f]) / P.xsd[f]);
+    if (P.X) P.X[c * F + f] = feature_norm(x, P.xmu[f], P.xsd[f]);
+  }
+}
+
+// Box path, first pass: sources are applied first and exactly, y* = rhoY0 + dt src,
+// rho h* = rho h0 + dt src_h, T* = HY2T(h*, Y*) from T0; the network then sees (Y*, T*, dt) and
+// predicts the chemistry over dt from y* (so fast modes end re-equilibrated, as in the full
+// problem). Writes the normalised features and T*.
+struct PrepParams {
+  int lo0, lo1, lo2, nx, ny;
+  long ncells;
+  Arr rY, rYsrc, T, rH, rHsrc;
+  IArr mask;
+  int has_mask;
+  double dt;
+  const float* xmu;
+  const float* xsd;
+  __bf16* X;
+  double* Tstar;
+  double* scale;  // [ncells x (N + 1)]: dt |f_k(y*)| / rho* (species), dt |f_T(y*)| (T)
+  const unsigned char* blob;
+  int blob_bytes, neff, ninc;
+};
+
+template <int N, int R, int G>
+__global__ void __launch_bounds__(BLOCK) prep_kernel(PrepParams P) {
+  constexpr int RPL = Cell<N, R, G>::RPL;
+  extern __shared__ __align__(16) unsigned char smem[];
+  {
+    const uint4* srcb = reinterpret_cast<const uint4*>(P.blob);
+    uint4* dst = reinterpret_cast<uint4*>(smem);
+    for (int b = threadIdx.x; b < P.blob_bytes / 16; b += blockDim.x) dst[b] = srcb[b];
+  }
+  __syncthreads();
+  const BlobLayout BL = blob_layout(N, R, P.neff, P.ninc);
+  Smem<N, R> S;
+  const int lane = threadIdx.x % G;
+  const int grp = threadIdx.x / G;
+  setup_views<N, R, G>(S, smem, BL, grp, nullptr);
+  const double dt = P.dt;
+  const int F = N + 2;
+  const long stride = (long)gridDim.x * (blockDim.x / G);
+  for (long c = (long)blockIdx.x * (blockDim.x / G) + grp; c < P.ncells; c += stride) {
+    const unsigned u = (unsigned)c;
+    const int i = P.lo0 + (int)(u % (unsigned)P.nx);
+    const int j = P.lo1 + (int)((u / (unsigned)P.nx) % (unsigned)P.ny);
+    const int k = P.lo2 + (int)(u / ((unsigned)P.nx * (unsigned)P.ny));
+    if (P.has_mask) {
+      const long mo = (long)(i - P.mask.bx) + (long)(j - P.mask.by) * P.mask.js + (long)(k - P.mask.bz) * P.mask.ks;
+      if (P.mask.p[mo] == -1) {
+        for (int f = lane; f < F; f += G) P.X[c * F + f] = (__bf16)0.0f;
+        continue;
+      }
+    }
+    const long offY = aoff(P.rY, i, j, k), offS = aoff(P.rYsrc, i, j, k);
+    double ys[RPL], Ys[RPL];
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      const int ii = lane + G * s;
+      ys[s] = (ii < N) ? P.rY.p[offY + (long)ii * P.rY.ns] + dt * P.rYsrc.p[offS + (long)ii * P.rYsrc.ns] : 0.0;
+    }
+    const double rhos = gpsum<G, RPL, N>(lane, ys);
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) Ys[s] = (lane + G * s < N) ? ys[s] / rhos : 0.0;
+    const double hs = (P.rH.p[aoff(P.rH, i, j, k)] + dt * P.rHsrc.p[aoff(P.rHsrc, i, j, k)]) / rhos;
+    const double Ts = hy2t<N, R, G>(S, lane, Ys, hs, P.T.p[aoff(P.T, i, j, k)]);
+    {
+      // chemistry-only rates at y* (no sources): the output scales
+      double yv[RPL], z[RPL], f[RPL];
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) {
+        yv[s] = ys[s];
+        z[s] = 0.0;
+      }
+      if (lane == N % G) yv[N / G] = Ts;
+      rhs_eval<N, R, G>(S, lane, 0.0, yv, z, hs * rhos, 0.0, f);
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) {
+        const int ii = lane + G * s;
+        if (ii < N) P.scale[c * (long)(N + 1) + ii] = dt * fabs(f[s]) / rhos;
+        else if (ii == N) P.scale[c * (long)(N + 1) + N] = dt * fabs(f[s]);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      const int ii = lane + G * s;
+      if (ii < N) P.X[c * F + ii] = feature_norm(feature_value(ii, N, Ys[s], Ts, dt), P.xmu[ii], P.xsd[ii]);
+    }
+    if (lane == 0) {
+      P.X[c * F + N] = feature_norm(feature_value(N, N, 0.0, Ts, dt), P.xmu[N], P.xsd[N]);
+      P.X[c * F + N + 1] = feature_norm(feature_value(N + 1, N, 0.0, Ts, dt), P.xmu[N + 1], P.xsd[N + 1]);
+      P.Tstar[c] = Ts;
+    }
   }
 }
 
@@ -90,6 +190,9 @@ struct TrustParams {
   const float* Yout;  // network output [ncells x (N + 1)], normalised
   const float* ymu;
   const float* ysd;
+  const float* yeps;    // rate-scale floors (0: the output is the increment itself)
+  const double* Tstar;  // T after the source step (prep kernel)
+  const double* scale;  // [ncells x (N + 1)] rate scales dt |f_i(y*)| (/ rho* for species)
   const double* Mel;  // [E x N] elements per gram of species k
   const double* Minv; // [E x E] (M M^T)^-1
   int E;
@@ -103,6 +206,13 @@ struct TrustParams {
   double* aud_out;  // [cap x (N + 1)] surrogate rhoY1, T1
   int* aud_cell;
 };
+
+// network output -> increment: u = o sd + mu; d = u (scale + eps) with the cell's rate scale
+// (eps > 0), or d = u (eps = 0)
+__device__ __forceinline__ double decode(float o, float mu, float sd, float eps, double scale) {
+  const double u = (double)o * (double)sd + (double)mu;
+  return eps > 0.f ? u * (scale + (double)eps) : u;
+}
 
 __device__ __forceinline__ unsigned int mix32(unsigned int x) {
   x ^= x >> 16;
@@ -158,13 +268,13 @@ __global__ void __launch_bounds__(BLOCK) trust_kernel(TrustParams P) {
       if (ii < N) {
         y0[s] = P.rY.p[offY + (long)ii * P.rY.ns];
         src[s] = P.rYsrc.p[offS + (long)ii * P.rYsrc.ns];
-        dY[s] = (double)o[ii] * (double)P.ysd[ii] + (double)P.ymu[ii];
+        dY[s] = decode(o[ii], P.ymu[ii], P.ysd[ii], P.yeps[ii], P.scale[c * (long)n + ii]);
       }
     }
     const double T0 = P.T.p[aoff(P.T, i, j, k)];
     const double rhoh0 = P.rH.p[aoff(P.rH, i, j, k)];
     const double srch = P.rHsrc.p[aoff(P.rHsrc, i, j, k)];
-    const double dT = (double)o[N] * (double)P.ysd[N] + (double)P.ymu[N];
+    const double dT = decode(o[N], P.ymu[N], P.ysd[N], P.yeps[N], P.scale[c * (long)n + N]);
     // element balance of the increments and its least-squares removal
     double a[SURR_MAXE], gross[SURR_MAXE];
 #pragma unroll
@@ -198,12 +308,16 @@ __global__ void __launch_bounds__(BLOCK) trust_kernel(TrustParams P) {
         if (ii < N) dY[s] -= P.Mel[e * N + ii] * lam;
       }
     }
-    const double rho0 = gpsum<G, RPL, N>(lane, y0);
+    // y* = y0 + dt src (sources first), then the chemistry increment at constant rho*
+    double ys[RPL];
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) ys[s] = (lane + G * s < N) ? y0[s] + dt * src[s] : 0.0;
+    const double rhos = gpsum<G, RPL, N>(lane, ys);
     double y1[RPL], Y1[RPL];
 #pragma unroll
     for (int s = 0; s < RPL; ++s) {
       const int ii = lane + G * s;
-      y1[s] = (ii < N) ? rho0 * (y0[s] / rho0 + dY[s]) + dt * src[s] : 0.0;
+      y1[s] = (ii < N) ? rhos * (ys[s] / rhos + dY[s]) : 0.0;
       ok = ok && (ii >= N || y1[s] >= -P.trust_atol);
     }
     const double rho1 = gpsum<G, RPL, N>(lane, y1);
@@ -213,8 +327,9 @@ __global__ void __launch_bounds__(BLOCK) trust_kernel(TrustParams P) {
       const int ii = lane + G * s;
       Y1[s] = (ii < N) ? y1[s] / rho1 : 0.0;
     }
-    double Tg = T0 + dT;
-    if (!(Tg > 200.0 && Tg < 6000.0)) Tg = T0;
+    const double Ts = P.Tstar[c];
+    double Tg = Ts + dT;
+    if (!(Tg > 200.0 && Tg < 6000.0)) Tg = Ts;
     const double T1 = hy2t<N, R, G>(S, lane, Y1, rhoh1 / rho1, Tg);
     ok = ok && (T1 > 200.0 && T1 < 6000.0);
     if (lane == N % G) y1[N / G] = T1;
@@ -236,7 +351,7 @@ __global__ void __launch_bounds__(BLOCK) trust_kernel(TrustParams P) {
     unsigned int slot = 0;
     if (accept) {
       const long offT = aoff(P.T, i, j, k);
-      const bool audit = mix32((unsigned)c ^ P.audit_seed) < P.audit_thresh;
+      const bool audit = P.audit_thresh == 0xFFFFFFFFu || mix32((unsigned)c ^ P.audit_seed) < P.audit_thresh;
       if (audit) {
         if (lane == 0) slot = atomicAdd(&P.counters[1], 1u);
         slot = __shfl(slot, 0, G);

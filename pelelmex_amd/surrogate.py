@@ -163,14 +163,16 @@ class MLP:
 
 def generate_pairs(reactor, seeds, dts, nx=64, device="cuda", log=print):
     """(state, result) pairs from the GPU BDF on flame-box states without external sources.
-    Returns device tensors rY0 [N, P], T0 [P], dt [P], rY1 [N, P], T1 [P]."""
+    Returns device tensors rY0 [N, P], T0 [P], dt [P], rY1 [N, P], T1 [P] and the BDF's right-hand
+    side evaluations per cell nfe [P] (its cost)."""
     import torch
     from .reactor import box_tensors
-    out = {k: [] for k in ("rY0", "T0", "dt", "rY1", "T1")}
+    out = {k: [] for k in ("rY0", "T0", "rH0", "dt", "rY1", "T1", "nfe")}
     for seed, dt in zip(seeds, dts):
         box = workloads.ch4_air_flame_box(nx=nx, ny=nx, nz=nx, seed=seed, dt=dt, src_frac=0.0)
         t = box_tensors(box, device)
         rY0, T0 = t["rY"].reshape(reactor.nsp, -1).clone(), t["T"].reshape(-1).clone()
+        rH0 = t["rH"].reshape(-1).clone()
         t0 = time.perf_counter()
         nfail, _, _ = reactor.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], None, dt, 0.0)
         torch.cuda.synchronize()
@@ -180,11 +182,14 @@ def generate_pairs(reactor, seeds, dts, nx=64, device="cuda", log=print):
             raise ReactorError(f"{nfail} cells failed while generating training pairs")
         out["rY0"].append(rY0)
         out["T0"].append(T0)
+        out["rH0"].append(rH0)
         out["dt"].append(torch.full_like(T0, dt))
         out["rY1"].append(t["rY"].reshape(reactor.nsp, -1).clone())
         out["T1"].append(t["T"].reshape(-1).clone())
-    return {"rY0": torch.cat(out["rY0"], 1), "T0": torch.cat(out["T0"]), "dt": torch.cat(out["dt"]),
-            "rY1": torch.cat(out["rY1"], 1), "T1": torch.cat(out["T1"])}
+        out["nfe"].append(t["FC"].reshape(-1).clone())
+    return {"rY0": torch.cat(out["rY0"], 1), "T0": torch.cat(out["T0"]), "rH0": torch.cat(out["rH0"]),
+            "dt": torch.cat(out["dt"]),
+            "rY1": torch.cat(out["rY1"], 1), "T1": torch.cat(out["T1"]), "nfe": torch.cat(out["nfe"])}
 
 
 def targets(pairs):
@@ -197,19 +202,57 @@ def targets(pairs):
     return torch.cat([dY, dT[None, :]], 0).T.contiguous()
 
 
-def train(reactor, pairs, hidden=HIDDEN, epochs=20, batch=4096, lr=1e-3, seed=0, log=print, time_budget=None):
-    """Trains an MLP on the pairs with the library's kernels; returns (mlp, xnorm, ynorm, history)."""
+def rate_scales(reactor, pairs, eps_Y=1e-9, eps_T=1e-3):
+    """Per-pair output scales dt |f_i(y0)| (/ rho0 for species) from the source-free reactor
+    right-hand side (the rates the library's prep kernel evaluates at inference), [P, N + 1],
+    and the floors eps [N + 1]."""
+    import torch
+    N = reactor.nsp
+    y0 = torch.cat([pairs["rY0"], pairs["T0"][None, :]], 0).contiguous()
+    f = reactor.rhs_batch(y0, torch.zeros_like(pairs["rY0"]), pairs["rH0"].contiguous(),
+                          torch.zeros_like(pairs["T0"]), 0.0)
+    s = f.abs() * pairs["dt"][None, :]
+    s[:N] = s[:N] / pairs["rY0"].sum(0)[None, :]
+    eps = torch.full((N + 1,), eps_Y, dtype=torch.float64, device=s.device)
+    eps[N] = eps_T
+    return s.T.contiguous(), eps
+
+
+def scaled_targets(reactor, pairs, umax=None):
+    """Targets u = d / (scale + eps) and eps; with umax, also the mask of pairs whose every
+    |u_i| <= umax (pairs far outside the explicit estimate, such as ignition, are left out)."""
+    import torch
+    s, eps = rate_scales(reactor, pairs)
+    u = targets(pairs) / (s + eps[None, :])
+    keep = (u.abs() <= umax).all(1) if umax is not None else torch.ones(u.shape[0], dtype=torch.bool, device=u.device)
+    return u, eps, keep
+
+
+def select_pairs(pairs, keep):
+    """The pairs where the boolean device tensor `keep` holds."""
+    return {k: (v[:, keep] if v.dim() == 2 else v[keep]) for k, v in pairs.items()}
+
+
+def train(reactor, pairs, hidden=HIDDEN, epochs=20, batch=4096, lr=1e-3, seed=0, log=print, time_budget=None,
+          scaled=True):
+    """Trains an MLP on the pairs with the library's kernels; returns (mlp, xnorm, ynorm, history).
+    Targets are the increments relative to the rate scales (scaled=True) or the increments
+    themselves, z-normalised."""
     import torch
     dev = pairs["T0"].device
     N = reactor.nsp
     raw = reactor.surrogate_features(pairs["rY0"].contiguous(), pairs["T0"], pairs["dt"])
     xmu = raw.double().mean(0)
     xsd = raw.double().std(0).clamp_min(1e-3)
-    tgt = targets(pairs)
+    if scaled:
+        tgt, eps, _ = scaled_targets(reactor, pairs)
+    else:
+        tgt = targets(pairs)
+        eps = torch.zeros(tgt.shape[1], dtype=torch.float64, device=dev)
     ymu = tgt.mean(0)
     ysd = tgt.std(0).clamp_min(1e-12)
     xnorm = torch.stack([xmu, xsd]).float().cpu().numpy()
-    ynorm = torch.stack([ymu, ysd]).float().cpu().numpy()
+    ynorm = torch.stack([ymu, ysd, eps]).float().cpu().numpy()
     X = ((raw - xmu.float()) / xsd.float()).to(torch.bfloat16).contiguous()
     Y = ((tgt - ymu) / ysd).float().contiguous()
     P = X.shape[0]
@@ -261,3 +304,104 @@ def attach(reactor, model):
 
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "surrogate_drm19.npz")
+
+
+def calibrate(reactor_keys, model, seed=40100, target=5e-5, nx=64, device="cuda", log=print,
+              grid_rtol=(1e-1, 3e-2, 1e-2, 3e-3, 1e-3, 3e-4, 1e-4), grid_atol=(1e-8, 3e-9, 1e-9, 3e-10, 1e-10, 1e-11)):
+    """Trust tolerances for a trained network: on a held-out flame box with sources, the network
+    result of every cell is compared with the BDF's, the trust score (WRMS backward-Euler
+    residual, the trust kernel's rule) is evaluated for each (trust_rtol, trust_atol) of the grid,
+    and the pair accepting the most BDF work with every accepted cell within `target` relative
+    temperature error is returned (None if no pair qualifies), with the sweep."""
+    import torch
+    from .reactor import box_tensors
+    box = workloads.ch4_air_flame_box(nx=nx, ny=nx, nz=nx, seed=seed)
+    t = box_tensors(box, device)
+    pr = {k: t[k].clone() for k in ("rY", "T", "rH")}
+    N = t["rY"].shape[0]
+
+    def run(rr):
+        for k in pr:
+            t[k].copy_(pr[k])
+        t["FC"].zero_()
+        nf, _, _ = rr.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], t["mask"], box["dt"], 0.0)
+        torch.cuda.synchronize()
+        return nf, t["rY"].clone(), t["T"].clone(), t["FC"].clone()
+
+    ref = Reactor.create("ReactorCvode", "drm19", reactor_keys)
+    ref.init(2, 1)
+    _, rY_ref, T_ref, FC_ref = run(ref)
+    loose = dict(reactor_keys, **{"surrogate.trust_rtol": 1e30, "surrogate.trust_atol": 1e30,
+                                  "surrogate.elem_tol": 1e30, "surrogate.audit_frac": 0.0})
+    rs = Reactor.create("ReactorCvode", "drm19", loose)
+    rs.init(2, 1)
+    attach(rs, model)
+    _, rY, T, FC = run(rs)
+    y0 = torch.cat([pr["rY"].reshape(N, -1), pr["T"].reshape(1, -1)], 0)
+    y1 = torch.cat([rY.reshape(N, -1), T.reshape(1, -1)], 0)
+    f1 = ref.rhs_batch(y1.contiguous(), t["rYsrc"].reshape(N, -1).contiguous(), pr["rH"].reshape(-1).contiguous(),
+                       t["rHsrc"].reshape(-1).contiguous(), box["dt"])
+    res = y1 - y0 - box["dt"] * f1
+    errT = ((T - T_ref).abs() / T_ref).reshape(-1)
+    cand = FC.reshape(-1) == 1.0          # cells the network produced a finite, in-range state for
+    fc = FC_ref.reshape(-1).double()
+    sweep, best = [], None
+    for rt in grid_rtol:
+        for at in grid_atol:
+            w = 1.0 / (rt * y1.abs() + at)
+            a = cand & (torch.sqrt(((res * w) ** 2).mean(0)) <= 1.0) & (y1[:N] >= -at).all(0)
+            if not bool(a.any()):
+                continue
+            d = {"trust_rtol": rt, "trust_atol": at, "accepted": float(a.double().mean()),
+                 "max_errT": float(errT[a].max()), "bdf_work_share": float(fc[a].sum() / fc.sum())}
+            sweep.append(d)
+            if d["max_errT"] <= target and (best is None or d["bdf_work_share"] > best["bdf_work_share"]):
+                best = d
+    ref.close()
+    rs.close()
+    log(f"  calibration: best {best}")
+    return best, sweep
+
+
+def main():
+    """Trains the drm19 network on the GPU and calibrates its trust tolerances:
+    python -m pelelmex_amd.surrogate --out <path.npz> [--boxes 8 --epochs 20]."""
+    import argparse
+    import json
+    import torch
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--boxes", type=int, default=8)
+    ap.add_argument("--nx", type=int, default=64)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--nfe-max", type=float, default=200.0)
+    ap.add_argument("--umax", type=float, default=20.0)
+    ap.add_argument("--seed", type=int, default=40040)
+    ap.add_argument("--target", type=float, default=5e-5)
+    args = ap.parse_args()
+    log = lambda *a: print(*a, flush=True)
+    keys = {"ode.rtol": 1e-10, "ode.atol": 1e-10, "cvode.max_order": 4, "cvode.max_substeps": 10000}
+    r = Reactor.create("ReactorCvode", "drm19", keys)
+    r.init(2, 1)
+    dev = torch.device("cuda:0")
+    pairs = generate_pairs(r, [args.seed + i for i in range(args.boxes)], [5e-6] * args.boxes, nx=args.nx,
+                           device=dev, log=log)
+    _, _, ukeep = scaled_targets(r, pairs, umax=args.umax)
+    keep = (pairs["nfe"] <= args.nfe_max) & ukeep
+    log(f"pairs {pairs['T0'].numel()}, training on {int(keep.sum())} (BDF cost <= {args.nfe_max} RHS, |u| <= {args.umax})")
+    sub = select_pairs(pairs, keep)
+    del pairs
+    mlp, xn, yn, hist = train(r, sub, epochs=args.epochs, log=log)
+    best, sweep = calibrate(keys, (mlp, xn, yn), target=args.target, nx=args.nx, device=dev, log=log)
+    meta = {"loss": np.asarray(hist, np.float64), "train_pairs": int(keep.sum()), "train_seed": args.seed,
+            "boxes": args.boxes, "epochs": args.epochs, "nfe_max": args.nfe_max, "umax": args.umax,
+            "calibration": json.dumps({"best": best, "sweep": sweep, "target": args.target})}
+    if best is not None:
+        meta["trust_rtol"] = best["trust_rtol"]
+        meta["trust_atol"] = best["trust_atol"]
+    save(args.out, mlp, xn, yn, **meta)
+    log(f"saved {args.out}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,126 @@
+"""GPU tests of the learned map + trust check + BDF fallback path of pr_react_box.
+
+* cells the trust check rejects are integrated by the same BDF kernel from untouched inputs:
+  bit-identical to a BDF-only call; accepted cells stay within max |dT| / T <= 1e-4 of it;
+* the audit (the BDF re-solving sampled accepted cells) reports exactly the error a direct
+  comparison gives on those cells;
+* the trained network gains over the method without it: it takes cells off the BDF (fewer BDF
+  right-hand-side evaluations for the box) where an untrained network of the same shape is
+  trusted with none or fewer;
+* bad networks are refused by pr_surrogate_load.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from pelelmex_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+KEYS = {"ode.rtol": 1e-10, "ode.atol": 1e-10, "cvode.max_order": 4, "cvode.max_substeps": 10000}
+
+
+def _model():
+    from pelelmex_amd import surrogate as sg
+    if not os.path.exists(sg.WEIGHTS):
+        pytest.skip("no trained network in the package")
+    return sg.load(sg.WEIGHTS)
+
+
+def _keys(model, **extra):
+    k = dict(KEYS)
+    if "trust_rtol" in model:
+        k["surrogate.trust_rtol"] = float(model["trust_rtol"])
+        k["surrogate.trust_atol"] = float(model["trust_atol"])
+    k.update(extra)
+    return k
+
+
+def _run(keys, box, gpu, model=None):
+    import torch
+    from pelelmex_amd import surrogate as sg
+    from pelelmex_amd.reactor import Reactor, box_tensors
+    r = Reactor.create("ReactorCvode", "drm19", keys)
+    r.init(2, 1)
+    if model is not None:
+        sg.attach(r, model)
+    t = box_tensors(box, gpu)
+    rH0 = t["rH"].clone()
+    nf, _, _ = r.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], t["mask"], box["dt"], 0.0)
+    torch.cuda.synchronize()
+    st = r.stats()
+    r.close()
+    out = {k: t[k].cpu().numpy().reshape(t[k].shape[0], -1) if k == "rY" else t[k].cpu().numpy().reshape(-1)
+           for k in ("rY", "T", "rH", "FC")}
+    out["rH0"] = rH0.cpu().numpy().reshape(-1)
+    return nf, st, out
+
+
+def _box():
+    return W.ch4_air_flame_box(nx=64, ny=8, nz=4, seed=2002)
+
+
+def test_fallback_exact_and_accepted_within_tolerance(gpu):
+    model = _model()
+    box = _box()
+    nc = box["T"].size
+    nf0, _, ref = _run(KEYS, box, gpu)
+    nf1, st, sur = _run(_keys(model), box, gpu, model)
+    assert nf0 == 0 and nf1 == 0
+    acc = sur["FC"] == 1.0
+    assert st["sur_accepted"] == int(acc.sum()) and st["sur_accepted"] + st["sur_rejected"] == nc
+    rej = ~acc
+    for k in ("rY", "T", "FC"):
+        np.testing.assert_array_equal(sur[k][..., rej], ref[k][..., rej])
+    np.testing.assert_array_equal(sur["rH"], ref["rH"])
+    if acc.any():
+        dT = np.abs(sur["T"][acc] - ref["T"][acc]) / ref["T"][acc]
+        assert dT.max() <= 1e-4, dT.max()
+
+
+def test_audit_reports_the_direct_error(gpu):
+    model = _model()
+    box = _box()
+    _, _, ref = _run(KEYS, box, gpu)
+    _, st, sur = _run(_keys(model, **{"surrogate.audit_frac": 1.0}), box, gpu, model)
+    acc = sur["FC"] == 1.0
+    if not acc.any():
+        pytest.skip("network accepted no cell of this box")
+    assert st["sur_audited"] == int(acc.sum())
+    direct = np.max(np.abs(ref["T"][acc] - sur["T"][acc]) / ref["T"][acc])
+    assert st["audit_max_dT_rel"] == pytest.approx(direct, rel=1e-12, abs=1e-300)
+    rho_r = ref["rY"][:, acc].sum(0)
+    rho_s = sur["rY"][:, acc].sum(0)
+    dY = np.max(np.abs(ref["rY"][:, acc] / rho_r - sur["rY"][:, acc] / rho_s))
+    assert st["audit_max_dY"] == pytest.approx(dY, rel=1e-12, abs=1e-300)
+
+
+def test_trained_network_gains_over_untrained(gpu):
+    from pelelmex_amd import surrogate as sg
+    model = _model()
+    box = _box()
+    _, st0, ref = _run(KEYS, box, gpu)
+    _, st1, sur = _run(_keys(model), box, gpu, model)
+    untrained = sg.MLP([int(d) for d in model["dims"]], device=gpu, seed=11)
+    _, st2, _ = _run(_keys(model), box, gpu, (untrained, model["xnorm"], model["ynorm"]))
+    bdf_rhs = ref["FC"].sum()
+    sur_rhs = sur["FC"][sur["FC"] != 1.0].sum()
+    assert st1["sur_accepted"] >= 0.1 * box["T"].size
+    assert sur_rhs <= 0.95 * bdf_rhs
+    assert st2["sur_accepted"] < st1["sur_accepted"]
+
+
+def test_load_rejects_bad_networks(gpu):
+    from pelelmex_amd.reactor import Reactor, ReactorError
+    r = Reactor.create("ReactorCvode", "drm19", KEYS)
+    r.init(2, 1)
+    N = r.nsp
+    with pytest.raises(ReactorError):       # wrong input width
+        r.load_surrogate([N + 1, 8, N + 1], np.zeros((N + 1) * 8 + 8 + 8 * (N + 1) + N + 1), np.ones((2, N + 1)),
+                         np.ones((3, N + 1)))
+    with pytest.raises(ReactorError):       # zero standard deviation
+        xn = np.ones((2, N + 2))
+        xn[1, 3] = 0.0
+        r.load_surrogate([N + 2, 8, N + 1], np.zeros((N + 2) * 8 + 8 + 8 * (N + 1) + N + 1), xn, np.ones((3, N + 1)))
+    r.close()
