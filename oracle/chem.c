@@ -509,16 +509,19 @@ int orc_react(int mech, long ncells, double* rY, const double* rYsrc, double* T,
       if (status) status[c] = 0;
       continue;
     }
-    double y[ORC_NMAX], src[ORC_NMAX];
+    double y[ORC_NMAX], src[ORC_NMAX], neg[ORC_NMAX];
     for (int k = 0; k < N; ++k) {
       y[k] = rY[k * ncells + c];
       src[k] = rYsrc[k * ncells + c];
+      /* ode.clean_init_massfrac: integrate from the non-negative part, add the rest back */
+      neg[k] = (opts->clean_init_massfrac && y[k] < 0.0) ? y[k] : 0.0;
+      if (neg[k] < 0.0) y[k] = 0.0;
     }
     y[N] = T[c];
     orc_cell cell = {mech, rH[c], rHsrc[c], src};
     orc_bdf_stats st;
     int ier = orc_bdf_solve(n, 0.0, dt, y, orc_reactor_rhs, orc_reactor_jac, &cell, opts, &st);
-    for (int k = 0; k < N; ++k) rY[k * ncells + c] = y[k];
+    for (int k = 0; k < N; ++k) rY[k * ncells + c] = neg[k] < 0.0 ? y[k] + neg[k] : y[k];
     T[c] = y[N];
     rH[c] = rH[c] + dt * rHsrc[c];
     FC[c] = (double)st.nfe;

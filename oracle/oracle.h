@@ -34,6 +34,8 @@ typedef struct {
   const double* atol_vec;
   int max_order;        /* cvode.max_order, 1..5 */
   int max_steps;        /* cvode.max_substeps */
+  int clean_init_massfrac; /* ode.clean_init_massfrac (box react): integrate from max(rhoY, 0) and
+                              add the clipped part back to the result */
 } orc_bdf_opts;
 
 typedef struct {

@@ -21,7 +21,7 @@ JAC_FN = C.CFUNCTYPE(C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_d
 
 class BdfOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("atol_vec", C.POINTER(C.c_double)),
-                ("max_order", C.c_int), ("max_steps", C.c_int)]
+                ("max_order", C.c_int), ("max_steps", C.c_int), ("clean_init_massfrac", C.c_int)]
 
 
 class BdfStats(C.Structure):
@@ -152,7 +152,7 @@ class Mech:
         return J
 
     def react(self, rY, rYsrc, T, rH, rHsrc, mask, dt, rtol=1e-10, atol=1e-10, max_order=5,
-              max_steps=10000, nthreads=1, atol_vec=None):
+              max_steps=10000, nthreads=1, atol_vec=None, clean_init_massfrac=0):
         """Box react on component-major arrays (shape [N, ncells]); returns new arrays."""
         rY = np.array(rY, dtype=np.float64, order="C", copy=True)
         T = np.array(T, dtype=np.float64, copy=True)
@@ -166,7 +166,7 @@ class Mech:
         av = None
         if atol_vec is not None:
             av = np.ascontiguousarray(atol_vec, dtype=np.float64)
-        o = BdfOpts(rtol, atol, _p(av) if av is not None else None, max_order, max_steps)
+        o = BdfOpts(rtol, atol, _p(av) if av is not None else None, max_order, max_steps, int(clean_init_massfrac))
         nst = C.c_long(0)
         lib().orc_react(self.id, ncell, _p(rY), _p(rYsrc), _p(T), _p(rH), _p(rHsrc), _p(FC),
                         _ip(mask), float(dt), C.byref(o), int(nthreads), _ip(status), C.byref(nst))

@@ -64,6 +64,7 @@ struct KParams {
   int* status;                   // per cell (linear box index), nullable
   unsigned int* nfail;
   double* gscr;                  // per-group global scratch slots (saved Jacobian, Jacobian records)
+  int clean;                     // ode.clean_init_massfrac: integrate from max(rhoY, 0), add the rest back
   const int* cell_list;          // when set: queue entry q integrates box cell cell_list[q]
   const unsigned int* ncells_dev;  // when set: min(*ncells_dev, ncells) entries are integrated
 };
@@ -161,6 +162,30 @@ __device__ __forceinline__ double rlane_d(double v, int l) {
   return __hiloint2double(hi, lo);
 }
 
+// value of lane ^ 16 (v_permlane16_swap exchanges rows 0 <-> 1 and 2 <-> 3)
+__device__ __forceinline__ double xor16_d(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  const bool up = (threadIdx.x & 16) != 0;
+  return __hiloint2double(up ? b[0] : b[1], up ? a[0] : a[1]);
+}
+
+// Two cells per wavefront (G == 32): lane l of the group is wave lane l (lanes 0..31) or
+// 32 + l (lanes 32..63). hsel picks this lane's half of a pair of wave-uniform values.
+__device__ __forceinline__ bool upper_half() { return (threadIdx.x & 32) != 0; }
+__device__ __forceinline__ double hsel(double lo, double hi) { return upper_half() ? hi : lo; }
+
+// butterfly over the 32 lanes of each half: DPP within rows, permlane16 swap across rows
+template <bool MAX>
+__device__ __forceinline__ double half_reduce(double v) {
+  v = MAX ? fmax(v, dpp_d<0xB1>(v)) : v + dpp_d<0xB1>(v);
+  v = MAX ? fmax(v, dpp_d<0x4E>(v)) : v + dpp_d<0x4E>(v);
+  v = MAX ? fmax(v, dpp_d<0x141>(v)) : v + dpp_d<0x141>(v);
+  v = MAX ? fmax(v, dpp_d<0x140>(v)) : v + dpp_d<0x140>(v);
+  return MAX ? fmax(v, xor16_d(v)) : v + xor16_d(v);
+}
+
 // wave-uniform copy of a value that is identical in all lanes
 template <int G>
 __device__ __forceinline__ int uni(int x) {
@@ -170,8 +195,14 @@ __device__ __forceinline__ int uni(int x) {
 
 template <int G>
 __device__ __forceinline__ double gbcast(double v, int src) {
-  if constexpr (G == 64) return rlane_d(v, __builtin_amdgcn_readfirstlane(src));
-  else return __shfl(v, src, G);
+  if constexpr (G == 64) {
+    return rlane_d(v, __builtin_amdgcn_readfirstlane(src));
+  } else if constexpr (G == 32) {
+    const int s0 = __builtin_amdgcn_readlane(src, 0), s1 = __builtin_amdgcn_readlane(src, 32) + 32;
+    return hsel(rlane_d(v, s0), rlane_d(v, s1));
+  } else {
+    return __shfl(v, src, G);
+  }
 }
 
 template <int G>
@@ -194,6 +225,8 @@ __device__ __forceinline__ double gmax(double v) {
     v = fmax(v, dpp_d<0x141>(v));  // quad-uniform values: the mirrored quad is lane ^ 4
     v = fmax(v, dpp_d<0x140>(v));  // 8-uniform values: the mirrored half row is lane ^ 8
     return fmax(fmax(rlane_d(v, 0), rlane_d(v, 16)), fmax(rlane_d(v, 32), rlane_d(v, 48)));
+  } else if constexpr (G == 32) {
+    return half_reduce<true>(v);
   } else {
 #pragma unroll
     for (int m = G / 2; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, G));
@@ -225,8 +258,12 @@ __device__ __forceinline__ double gpsum(int lane, const double (&a)[RPL]) {
   for (int s = 0; s < NB; ++s) {
     double v = 0.0;
     if (s < RPL) v = (lane + G * s < CNT) ? a[s < RPL ? s : 0] : 0.0;
+    if constexpr (G == 32) {
+      v = half_reduce<false>(v);  // the same pairwise tree over the 32 lanes of the half
+    } else {
 #pragma unroll
-    for (int m = 1; m < G; m <<= 1) v += __shfl_xor(v, m, G);
+      for (int m = 1; m < G; m <<= 1) v += __shfl_xor(v, m, G);
+    }
     t[s] = v;
   }
 #pragma unroll
@@ -733,6 +770,42 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
     }
     return singular;
   }
+  if constexpr (G == 32 && RPL == 1) {
+    // as above, one pivot owner per half; the pivot row is read with readlane from both halves
+    pst[0] = (lane < n) ? UNPIV : -1;
+    rinv[0] = 0.0;
+#pragma unroll
+    for (int j = 0; j < n; ++j) lu[0][j] = (lane < n) ? ((lane == j ? 1.0 : 0.0) - gamma * S.Js[lane * n + j]) : 0.0;
+    bool singular = false;
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+      double cand = -1.0;
+      if (pst[0] == UNPIV) {
+        const double v = fabs(lu[0][k]);
+        cand = (v == v) ? v : -2.0;
+      }
+      const double best = gmax<32>(cand);
+      if (!(best > 0.0)) {
+        singular = true;
+        break;
+      }
+      const unsigned long long m = __ballot(cand == best);
+      const unsigned int m0 = (unsigned int)m, m1 = (unsigned int)(m >> 32);
+      const int o0 = m0 ? __builtin_ctz(m0) : 0, o1 = (m1 ? __builtin_ctz(m1) : 0) + 32;  // first maximal rows
+      const int owner = upper_half() ? o1 - 32 : o0;
+      if (lane == owner) pst[0] = k;
+      if (lane == k) permv = owner;
+      const double inv = 1.0 / hsel(rlane_d(lu[0][k], o0), rlane_d(lu[0][k], o1));
+      if (lane == owner) rinv[0] = inv;
+      if (pst[0] > k) lu[0][k] *= inv;
+#pragma unroll
+      for (int j = k + 1; j < n; ++j) {
+        const double pj = hsel(rlane_d(lu[0][j], o0), rlane_d(lu[0][j], o1));
+        if (pst[0] > k) lu[0][j] = fma(-lu[0][k], pj, lu[0][j]);
+      }
+    }
+    return singular;
+  }
 #pragma unroll
   for (int s = 0; s < RPL; ++s) {
     const int i = lane + G * s;
@@ -813,6 +886,23 @@ __device__ __forceinline__ void lu_solve(const Smem<N, R>& S, int lane, const do
     for (int k = n - 1; k >= 0; --k) {
       const int p = __builtin_amdgcn_readlane(permv, k);
       const double xk = rlane_d(b[0], p) * rlane_d(rinv[0], p);
+      if (pst[0] >= 0 && pst[0] < k) b[0] = fma(-lu[0][k], xk, b[0]);
+      if (lane == k) x[0] = xk;
+    }
+    return;
+  }
+  if constexpr (G == 32 && RPL == 1) {
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+      const int p0 = __builtin_amdgcn_readlane(permv, k), p1 = __builtin_amdgcn_readlane(permv, 32 + k) + 32;
+      const double yk = hsel(rlane_d(b[0], p0), rlane_d(b[0], p1));
+      if (pst[0] > k) b[0] = fma(-lu[0][k], yk, b[0]);
+    }
+    x[0] = 0.0;
+#pragma unroll
+    for (int k = n - 1; k >= 0; --k) {
+      const int p0 = __builtin_amdgcn_readlane(permv, k), p1 = __builtin_amdgcn_readlane(permv, 32 + k) + 32;
+      const double xk = hsel(rlane_d(b[0], p0) * rlane_d(rinv[0], p0), rlane_d(b[0], p1) * rlane_d(rinv[0], p1));
       if (pst[0] >= 0 && pst[0] < k) b[0] = fma(-lu[0][k], xk, b[0]);
       if (lane == k) x[0] = xk;
     }

@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
   c.phase = PH_FETCH;
 
   // vectors kept in registers (rows lane, lane + G, ...)
-  double zn[Q + 1][RPL], ewt[RPL], acor[RPL], yv[RPL], f[RPL], src[RPL], atol_r[RPL];
+  double zn[Q + 1][RPL], ewt[RPL], acor[RPL], yv[RPL], f[RPL], src[RPL], atol_r[RPL], neg[RPL];
   double lu[RPL][n];
   int pst[RPL];
   double rinv[RPL];
@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
 #pragma unroll
     for (int s = 0; s < RPL; ++s) {
       int ii = lane + G * s;
-      if (ii < N) P.rY.p[offY + (long)ii * P.rY.ns] = out[s];
+      if (ii < N) P.rY.p[offY + (long)ii * P.rY.ns] = neg[s] < 0.0 ? out[s] + neg[s] : out[s];
       else if (ii == N) P.T.p[offT] = out[s];
     }
     if (lane == 0) {
@@ -398,9 +398,14 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
         int ii = lane + G * s;
         double y0 = 0.0;
         src[s] = 0.0;
+        neg[s] = 0.0;
         if (ii < N) {
           y0 = P.rY.p[offY + (long)ii * P.rY.ns];
           src[s] = P.rYsrc.p[offS + (long)ii * P.rYsrc.ns];
+          if (P.clean && y0 < 0.0) {  // integrate from the non-negative part
+            neg[s] = y0;
+            y0 = 0.0;
+          }
         } else if (ii == N) {
           y0 = P.T.p[aoff(P.T, i, j, k)];
         }

@@ -382,8 +382,8 @@ int pr_create(const char* integrator, const char* mechanism, int reactor_type, i
   Keys K;
   int rc = parse_keys(keyvals, K);
   if (rc) return rc;
-  if (K.clean_init_massfrac != 0)
-    return fail(PR_ERR_UNSUPPORTED, "ode.clean_init_massfrac = 1 is not implemented");
+  if (K.clean_init_massfrac != 0 && K.clean_init_massfrac != 1)
+    return fail(PR_ERR_KEY, "ode.clean_init_massfrac must be 0 or 1");
   int G = K.lanes_per_cell;
   if (G == 0) G = (m->nsp + 1 <= 16) ? 16 : (m->nsp + 1 <= 32 ? 32 : 64);
   const Variant* v = find_variant(m->name, G);
@@ -620,6 +620,7 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   P.atolv = h->use_typ ? h->d_atolv : nullptr;
   P.max_order = h->keys.max_order;
   P.max_steps = h->keys.max_substeps;
+  P.clean = h->keys.clean_init_massfrac;
   P.blob = h->d_blob;
   P.blob_bytes = h->blob_bytes;
   P.neff = h->mech->neff;
@@ -657,6 +658,7 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
     F.rHsrc = P.rHsrc;
     F.mask = P.mask;
     F.has_mask = P.has_mask;
+    F.clean = P.clean;
     F.dt = dt;
     F.xmu = S->xmu;
     F.xsd = S->xsd;
@@ -685,6 +687,7 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
     Q.FC = P.FC;
     Q.mask = P.mask;
     Q.has_mask = P.has_mask;
+    Q.clean = P.clean;
     Q.status = status_out;
     Q.dt = dt;
     Q.Yout = S->Yout;

@@ -9,7 +9,7 @@
 // z-normalised: frozen cells give d = 0 and near-equilibrium cells O(1) targets. The trust kernel then
 //  * removes the increments' element imbalance by the least-squares projection
 //    dY <- dY - M^T (M M^T)^-1 M dY, M_ek = a_ek / W_k (elements per gram), and rejects a
-//    cell whose raw imbalance exceeds elem_tol of its gross change,
+//    cell whose raw imbalance exceeds elem_tol of its gross change plus 1e-10 mol / g,
 //  * applies the sources first and exactly (prep kernel): y* = rhoY0 + dt src, rhoh1 =
 //    rhoh0 + dt src_h, T* = HY2T(rhoh1 / rho*, Y*); the network sees (Y*, T*, dt), and the cell
 //    becomes rhoY1 = rho* (Y* + dY), T1 = HY2T(rhoh1 / rho1, Y1) started at T* + dT,
@@ -97,6 +97,7 @@ struct PrepParams {
   Arr rY, rYsrc, T, rH, rHsrc;
   IArr mask;
   int has_mask;
+  int clean;  // ode.clean_init_massfrac
   double dt;
   const float* xmu;
   const float* xsd;
@@ -142,7 +143,9 @@ __global__ void __launch_bounds__(BLOCK) prep_kernel(PrepParams P) {
 #pragma unroll
     for (int s = 0; s < RPL; ++s) {
       const int ii = lane + G * s;
-      ys[s] = (ii < N) ? P.rY.p[offY + (long)ii * P.rY.ns] + dt * P.rYsrc.p[offS + (long)ii * P.rYsrc.ns] : 0.0;
+      double y0 = (ii < N) ? P.rY.p[offY + (long)ii * P.rY.ns] : 0.0;
+      if (P.clean && y0 < 0.0) y0 = 0.0;
+      ys[s] = (ii < N) ? y0 + dt * P.rYsrc.p[offS + (long)ii * P.rYsrc.ns] : 0.0;
     }
     const double rhos = gpsum<G, RPL, N>(lane, ys);
 #pragma unroll
@@ -185,6 +188,7 @@ struct TrustParams {
   Arr rY, rYsrc, T, rH, rHsrc, FC;
   IArr mask;
   int has_mask;
+  int clean;  // ode.clean_init_massfrac: the cell starts from max(rhoY, 0); the rest is added back
   int* status;
   double dt;
   const float* Yout;  // network output [ncells x (N + 1)], normalised
@@ -257,7 +261,7 @@ __global__ void __launch_bounds__(BLOCK) trust_kernel(TrustParams P) {
       }
     }
     const long offY = aoff(P.rY, i, j, k), offS = aoff(P.rYsrc, i, j, k);
-    double y0[RPL], src[RPL], dY[RPL];
+    double y0[RPL], src[RPL], dY[RPL], neg[RPL];
     const float* o = P.Yout + c * (long)n;
 #pragma unroll
     for (int s = 0; s < RPL; ++s) {
@@ -265,8 +269,13 @@ __global__ void __launch_bounds__(BLOCK) trust_kernel(TrustParams P) {
       y0[s] = 0.0;
       src[s] = 0.0;
       dY[s] = 0.0;
+      neg[s] = 0.0;
       if (ii < N) {
         y0[s] = P.rY.p[offY + (long)ii * P.rY.ns];
+        if (P.clean && y0[s] < 0.0) {
+          neg[s] = y0[s];
+          y0[s] = 0.0;
+        }
         src[s] = P.rYsrc.p[offS + (long)ii * P.rYsrc.ns];
         dY[s] = decode(o[ii], P.ymu[ii], P.ysd[ii], P.yeps[ii], P.scale[c * (long)n + ii]);
       }
@@ -294,7 +303,7 @@ __global__ void __launch_bounds__(BLOCK) trust_kernel(TrustParams P) {
     }
     bool ok = true;
 #pragma unroll
-    for (int e = 0; e < SURR_MAXE; ++e) ok = ok && (e >= P.E || fabs(a[e]) <= P.elem_tol * fmax(gross[e], 1e-30));
+    for (int e = 0; e < SURR_MAXE; ++e) ok = ok && (e >= P.E || fabs(a[e]) <= P.elem_tol * gross[e] + 1e-10);
 #pragma unroll
     for (int e = 0; e < SURR_MAXE; ++e) {
       if (e >= P.E) continue;
@@ -362,10 +371,11 @@ __global__ void __launch_bounds__(BLOCK) trust_kernel(TrustParams P) {
           for (int s = 0; s < RPL; ++s) {
             const int ii = lane + G * s;
             if (ii < N) {
-              ai[ii] = y0[s];
+              ai[ii] = y0[s] + neg[s];  // the caller's input
               ai[N + ii] = src[s];
             }
-            if (ii < n) ao[ii] = y1[s];
+            if (ii < N) ao[ii] = neg[s] < 0.0 ? y1[s] + neg[s] : y1[s];
+            else if (ii == N) ao[ii] = y1[s];
           }
           if (lane == 0) {
             ai[2 * N] = T0;
@@ -378,7 +388,7 @@ __global__ void __launch_bounds__(BLOCK) trust_kernel(TrustParams P) {
 #pragma unroll
       for (int s = 0; s < RPL; ++s) {
         const int ii = lane + G * s;
-        if (ii < N) P.rY.p[offY + (long)ii * P.rY.ns] = y1[s];
+        if (ii < N) P.rY.p[offY + (long)ii * P.rY.ns] = neg[s] < 0.0 ? y1[s] + neg[s] : y1[s];
         else if (ii == N) P.T.p[offT] = y1[s];
       }
       if (lane == 0) {

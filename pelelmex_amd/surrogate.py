@@ -306,59 +306,61 @@ def attach(reactor, model):
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "surrogate_drm19.npz")
 
 
-def calibrate(reactor_keys, model, seed=40100, target=5e-5, nx=64, device="cuda", log=print,
-              grid_rtol=(1e-1, 3e-2, 1e-2, 3e-3, 1e-3, 3e-4, 1e-4), grid_atol=(1e-8, 3e-9, 1e-9, 3e-10, 1e-10, 1e-11)):
-    """Trust tolerances for a trained network: on a held-out flame box with sources, the network
-    result of every cell is compared with the BDF's, the trust score (WRMS backward-Euler
-    residual, the trust kernel's rule) is evaluated for each (trust_rtol, trust_atol) of the grid,
-    and the pair accepting the most BDF work with every accepted cell within `target` relative
-    temperature error is returned (None if no pair qualifies), with the sweep."""
+def calibrate(reactor_keys, model, seeds=(40100, 40101), target=1e-6, nx=64, device="cuda", log=print,
+              grid_rtol=(1e-1, 3e-2, 1e-2, 3e-3, 1e-3, 3e-4, 1e-4), grid_atol=(1e-9, 3e-10, 1e-10, 3e-11, 1e-11)):
+    """Trust tolerances for a trained network. On held-out flame boxes with sources (seeds),
+    pr_react_box runs the learned path with each (trust_rtol, trust_atol) of the grid and its
+    accepted cells (FC = 1) are compared with a BDF-only call. Among the pairs whose accepted
+    cells all stay within `target` relative temperature error on every box (the default is 100x
+    below the 1e-4 accuracy bound of the metric, a margin for boxes not seen here), the one that
+    takes the most BDF work off is chosen, preferring the smallest trust_atol, then trust_rtol,
+    among pairs within 2 % of that work. Returns (best or None, sweep)."""
     import torch
     from .reactor import box_tensors
-    box = workloads.ch4_air_flame_box(nx=nx, ny=nx, nz=nx, seed=seed)
-    t = box_tensors(box, device)
-    pr = {k: t[k].clone() for k in ("rY", "T", "rH")}
-    N = t["rY"].shape[0]
 
-    def run(rr):
+    def run(t, pr, box, keys, with_model):
+        rr = Reactor.create("ReactorCvode", "drm19", keys)
+        rr.init(2, 1)
+        if with_model:
+            attach(rr, model)
         for k in pr:
             t[k].copy_(pr[k])
         t["FC"].zero_()
-        nf, _, _ = rr.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], t["mask"], box["dt"], 0.0)
+        rr.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], t["mask"], box["dt"], 0.0)
         torch.cuda.synchronize()
-        return nf, t["rY"].clone(), t["T"].clone(), t["FC"].clone()
+        rr.close()
+        return t["T"].reshape(-1).clone(), t["FC"].reshape(-1).clone()
 
-    ref = Reactor.create("ReactorCvode", "drm19", reactor_keys)
-    ref.init(2, 1)
-    _, rY_ref, T_ref, FC_ref = run(ref)
-    loose = dict(reactor_keys, **{"surrogate.trust_rtol": 1e30, "surrogate.trust_atol": 1e30,
-                                  "surrogate.elem_tol": 1e30, "surrogate.audit_frac": 0.0})
-    rs = Reactor.create("ReactorCvode", "drm19", loose)
-    rs.init(2, 1)
-    attach(rs, model)
-    _, rY, T, FC = run(rs)
-    y0 = torch.cat([pr["rY"].reshape(N, -1), pr["T"].reshape(1, -1)], 0)
-    y1 = torch.cat([rY.reshape(N, -1), T.reshape(1, -1)], 0)
-    f1 = ref.rhs_batch(y1.contiguous(), t["rYsrc"].reshape(N, -1).contiguous(), pr["rH"].reshape(-1).contiguous(),
-                       t["rHsrc"].reshape(-1).contiguous(), box["dt"])
-    res = y1 - y0 - box["dt"] * f1
-    errT = ((T - T_ref).abs() / T_ref).reshape(-1)
-    cand = FC.reshape(-1) == 1.0          # cells the network produced a finite, in-range state for
-    fc = FC_ref.reshape(-1).double()
-    sweep, best = [], None
-    for rt in grid_rtol:
-        for at in grid_atol:
-            w = 1.0 / (rt * y1.abs() + at)
-            a = cand & (torch.sqrt(((res * w) ** 2).mean(0)) <= 1.0) & (y1[:N] >= -at).all(0)
-            if not bool(a.any()):
-                continue
-            d = {"trust_rtol": rt, "trust_atol": at, "accepted": float(a.double().mean()),
-                 "max_errT": float(errT[a].max()), "bdf_work_share": float(fc[a].sum() / fc.sum())}
-            sweep.append(d)
-            if d["max_errT"] <= target and (best is None or d["bdf_work_share"] > best["bdf_work_share"]):
-                best = d
-    ref.close()
-    rs.close()
+    results = {}
+    for seed in seeds:
+        box = workloads.ch4_air_flame_box(nx=nx, ny=nx, nz=nx, seed=seed)
+        t = box_tensors(box, device)
+        pr = {k: t[k].clone() for k in ("rY", "T", "rH")}
+        T_ref, FC_ref = run(t, pr, box, reactor_keys, False)
+        fc = FC_ref.double()
+        for rt in grid_rtol:
+            for at in grid_atol:
+                keys = dict(reactor_keys, **{"surrogate.trust_rtol": rt, "surrogate.trust_atol": at,
+                                             "surrogate.audit_frac": 0.0})
+                T, FC = run(t, pr, box, keys, True)
+                a = FC == 1.0
+                e = float(((T[a] - T_ref[a]).abs() / T_ref[a]).max()) if bool(a.any()) else 0.0
+                r = results.setdefault((rt, at), {"trust_rtol": rt, "trust_atol": at, "accepted": [], "max_errT": 0.0,
+                                                  "bdf_work_share": []})
+                r["accepted"].append(float(a.double().mean()))
+                r["bdf_work_share"].append(float(fc[a].sum() / fc.sum()))
+                r["max_errT"] = max(r["max_errT"], e)
+    sweep = []
+    for r in results.values():
+        r["accepted"] = float(np.mean(r["accepted"]))
+        r["bdf_work_share"] = float(np.mean(r["bdf_work_share"]))
+        sweep.append(r)
+    ok = [r for r in sweep if r["max_errT"] <= target and r["accepted"] > 0]
+    best = None
+    if ok:
+        top = max(r["bdf_work_share"] for r in ok)
+        near = [r for r in ok if r["bdf_work_share"] >= 0.98 * top]
+        best = min(near, key=lambda r: (r["trust_atol"], r["trust_rtol"]))
     log(f"  calibration: best {best}")
     return best, sweep
 
@@ -377,13 +379,26 @@ def main():
     ap.add_argument("--nfe-max", type=float, default=200.0)
     ap.add_argument("--umax", type=float, default=20.0)
     ap.add_argument("--seed", type=int, default=40040)
-    ap.add_argument("--target", type=float, default=5e-5)
+    ap.add_argument("--target", type=float, default=1e-6)
+    ap.add_argument("--model", default=None, help="calibrate this trained network instead of training one")
     args = ap.parse_args()
     log = lambda *a: print(*a, flush=True)
     keys = {"ode.rtol": 1e-10, "ode.atol": 1e-10, "cvode.max_order": 4, "cvode.max_substeps": 10000}
     r = Reactor.create("ReactorCvode", "drm19", keys)
     r.init(2, 1)
     dev = torch.device("cuda:0")
+    if args.model:
+        m = load(args.model)
+        best, sweep = calibrate(keys, m, target=args.target, nx=args.nx, device=dev, log=log)
+        meta = {k: v for k, v in m.items() if k not in ("dims", "params", "xnorm", "ynorm", "trust_rtol", "trust_atol",
+                                                       "calibration")}
+        meta["calibration"] = json.dumps({"best": best, "sweep": sweep, "target": args.target})
+        if best is not None:
+            meta["trust_rtol"] = best["trust_rtol"]
+            meta["trust_atol"] = best["trust_atol"]
+        np.savez(args.out, dims=m["dims"], params=m["params"], xnorm=m["xnorm"], ynorm=m["ynorm"], **meta)
+        log(f"saved {args.out}")
+        return
     pairs = generate_pairs(r, [args.seed + i for i in range(args.boxes)], [5e-6] * args.boxes, nx=args.nx,
                            device=dev, log=log)
     _, _, ukeep = scaled_targets(r, pairs, umax=args.umax)

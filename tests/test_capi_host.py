@@ -50,7 +50,10 @@ def _create(integrator=b"ReactorCvode", mech=b"drm19", rtype=2, keys=None):
     (b"ode.unknown_key = 1", -4),
     (b"hipbdf.block = 100", -4),
     (b"justtext", -4),
-    (b"ode.clean_init_massfrac = 1", -2),
+    (b"ode.clean_init_massfrac = 2", -4),
+    (b"surrogate.trust_rtol = 0", -4),
+    (b"surrogate.audit_frac = 1.5", -4),
+    (b"surrogate.unknown = 1", -4),
     (b"hipbdf.lanes_per_cell = 3", -2),
 ])
 def test_pr_create_key_errors(keys, code):
@@ -70,6 +73,8 @@ def test_pr_create_rejects_unsupported():
     assert _create(integrator=b"ReactorRK64")[0] == -2
     # keys of other namespaces pass through untouched; a key error is reported before any HIP call
     rc, _ = _create(keys=b"peleLM.chem_integrator = ReactorCvode\node.rtol = 1e-8 cvode.max_order=4")
+    assert rc in (0, -3)
+    rc, _ = _create(keys=b"ode.clean_init_massfrac = 1 surrogate.audit_frac = 0.02 surrogate.enable = 0")
     assert rc in (0, -3)
 
 
