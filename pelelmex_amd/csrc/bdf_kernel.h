@@ -430,6 +430,7 @@ struct Smem {
   double* sCpm;  // N mass-specific cp
   double* q;     // R rates of progress
   int* perm;     // N+1 pivot rows
+  double* LU;    // (N+1)^2 LU factors, row-major; rows live in registers only during factor / solve
   static constexpr int n = N + 1;
   // per-group LDS (doubles) and per-group global scratch slot (doubles)
   static constexpr int scratch_doubles = 5 * N + R + (n + 1) / 2 + 1;
@@ -444,7 +445,7 @@ struct Cell {
   // With one cell per wavefront the saved Jacobian and its records sit in LDS after the group's
   // scratch (few groups per block); narrower groups keep them in a global scratch slot.
   static constexpr bool js_lds = (G == 64);
-  static constexpr int lds_doubles = Smem<N, R>::scratch_doubles + (js_lds ? Smem<N, R>::slot_doubles : 0);
+  static constexpr int lds_doubles = Smem<N, R>::scratch_doubles + n * n + (js_lds ? Smem<N, R>::slot_doubles : 0);
   static constexpr int gslot_doubles = js_lds ? 0 : Smem<N, R>::slot_doubles;
 };
 
@@ -865,6 +866,31 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
   }
   group_sync();
   return singular;
+}
+
+// The LU rows persist in the group's LDS area between factorizations; registers hold them only
+// while a factorization or a solve runs.
+template <int N, int R, int G>
+__device__ __forceinline__ void store_lu(const Smem<N, R>& S, int lane, const double (&lu)[Cell<N, R, G>::RPL][N + 1]) {
+  constexpr int n = N + 1;
+#pragma unroll
+  for (int s = 0; s < Cell<N, R, G>::RPL; ++s) {
+    const int i = lane + G * s;
+    if (i < n)
+#pragma unroll
+      for (int j = 0; j < n; ++j) S.LU[i * n + j] = lu[s][j];
+  }
+}
+
+template <int N, int R, int G>
+__device__ __forceinline__ void load_lu(const Smem<N, R>& S, int lane, double (&lu)[Cell<N, R, G>::RPL][N + 1]) {
+  constexpr int n = N + 1;
+#pragma unroll
+  for (int s = 0; s < Cell<N, R, G>::RPL; ++s) {
+    const int i = lane + G * s;
+#pragma unroll
+    for (int j = 0; j < n; ++j) lu[s][j] = (i < n) ? S.LU[i * n + j] : 0.0;
+  }
 }
 
 // Solve M x = b with the factors from lu_factor; b is distributed by row, x by component.

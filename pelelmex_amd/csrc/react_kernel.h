@@ -61,8 +61,9 @@ __device__ __forceinline__ double* setup_views(Smem<N, R>& S, unsigned char* sme
   S.sCpm = S.sHm + N;
   S.q = S.sCpm + N;
   S.perm = reinterpret_cast<int*>(S.q + R);
+  S.LU = S.q + R + (n + 1) / 2 + 1;
   if constexpr (Cell<N, R, G>::js_lds) {
-    S.Js = S.q + R + (n + 1) / 2 + 1;
+    S.Js = S.LU + n * n;
     S.jrec = S.Js + n * n;
   } else if (gscr) {
     const long slot = (long)blockIdx.x * (blockDim.x / G) + grp;
@@ -106,7 +107,6 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
 
   // vectors kept in registers (rows lane, lane + G, ...)
   double zn[Q + 1][RPL], ewt[RPL], acor[RPL], yv[RPL], f[RPL], src[RPL], atol_r[RPL], neg[RPL];
-  double lu[RPL][n];
   int pst[RPL];
   double rinv[RPL];
   int permv = 0;
@@ -525,7 +525,11 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
         } else {
           c.jcur = 0;
         }
-        singular = lu_factor<N, R, G>(S, lane, c.gamma, lu, pst, rinv, permv);
+        {
+          double lu[RPL][n];
+          singular = lu_factor<N, R, G>(S, lane, c.gamma, lu, pst, rinv, permv);
+          store_lu<N, R, G>(S, lane, lu);
+        }
         clobber();
         c.nset += 1;
         c.gamrat = 1.0;
@@ -540,7 +544,11 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
         double b[RPL], dlt[RPL];
 #pragma unroll
         for (int s = 0; s < RPL; ++s) b[s] = -res[s];
-        lu_solve<N, R, G>(S, lane, lu, pst, rinv, permv, b, dlt);
+        {
+          double lu[RPL][n];
+          load_lu<N, R, G>(S, lane, lu);
+          lu_solve<N, R, G>(S, lane, lu, pst, rinv, permv, b, dlt);
+        }
         clobber();
         const double gamrat = c.gamrat;
         if (gamrat != 1.0) {

@@ -427,12 +427,13 @@ int pr_set_typ_vals(pr_handle* h, const double* typ, int n) {
 
 int pr_num_species(const pr_handle* h) { return h ? h->mech->nsp : PR_ERR_ARG; }
 
-// Largest block (64..256 threads) whose LDS (tables + per-group scratch) stays within 64 KiB.
+// Largest block (64..256 threads) whose LDS (tables + per-group scratch) stays within 80 KiB,
+// so that two blocks share a CU's 160 KiB.
 static int pick_block(const pr_handle* h) {
   if (h->block) return h->block;
   const int G = h->var->G;
   for (int b = 256; b >= 64; b -= 64)
-    if ((size_t)h->blob_bytes + (size_t)(b / G) * h->var->scratch <= 65536) return b;
+    if ((size_t)h->blob_bytes + (size_t)(b / G) * h->var->scratch <= 81920) return b;
   return 64;
 }
 
