@@ -613,12 +613,15 @@ __device__ __forceinline__ void rate_rhoY_eval(const Smem<N, R>& S, int lane, co
 }
 
 // ---------------------------------------------------------------- analytic Jacobian
-// Exact Jacobian of the right-hand side with T = y_T, written into S.Js (row-major).
+// Exact Jacobian of the right-hand side with T = y_T, written into S.Js (row-major). Rows are
+// assembled in LDS (the saved Jacobian itself at G = 64, else the LU area, which is dead here:
+// a factorization always follows) and then copied to the global slot.
 template <int N, int R, int G>
 __device__ __forceinline__ void jac_eval(const Smem<N, R>& S, int lane, const double (&y)[Cell<N, R, G>::RPL],
                                          const double (&src)[Cell<N, R, G>::RPL], double rhohsrc) {
   constexpr int RPL = Cell<N, R, G>::RPL;
   constexpr int n = N + 1;
+  double* const A = Cell<N, R, G>::js_lds ? S.Js : S.LU;  // assembly rows (LDS)
   const double T = gbcast<G>(y[N / G], N % G);
   const double lnT = pr_log(T), invT = 1.0 / T, lnRTP = pr_log((PR_RU * T) / PR_PATM);
   double cc[RPL], cpk[RPL], dcpk[RPL];
@@ -668,7 +671,7 @@ __device__ __forceinline__ void jac_eval(const Smem<N, R>& S, int lane, const do
     psd[s] = 0.0;
     pcf[s] = 0.0;
     if (k < N) {
-      double* row = S.Js + k * n;
+      double* row = A + k * n;
       for (int j = 0; j < n; ++j) row[j] = 0.0;
       double Sd = 0.0, JT = 0.0, w = 0.0;
       for (int e = S.incs[k]; e < S.incs[k + 1]; ++e) {
@@ -718,11 +721,15 @@ __device__ __forceinline__ void jac_eval(const Smem<N, R>& S, int lane, const do
     const int j = lane + G * s;
     if (j < n) {
       double cs = 0.0;
-      for (int k = 0; k < N; ++k) cs += S.sHm[k] * S.Js[k * n + j];
-      S.Js[N * n + j] = (j < N) ? (-cs - fT * S.sCpm[j]) / rcp : ((-cs - scf) - fT * sdcp) / rcp;
+      for (int k = 0; k < N; ++k) cs += S.sHm[k] * A[k * n + j];
+      A[N * n + j] = (j < N) ? (-cs - fT * S.sCpm[j]) / rcp : ((-cs - scf) - fT * sdcp) / rcp;
     }
   }
   group_sync();
+  if constexpr (!Cell<N, R, G>::js_lds) {
+    for (int e = lane; e < n * n; e += G) S.Js[e] = A[e];
+    group_sync();
+  }
 }
 
 // ---------------------------------------------------------------- dense LU and solve

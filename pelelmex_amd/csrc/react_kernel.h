@@ -191,32 +191,51 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
           for (int s = 0; s < RPL; ++s) zn[j - 1][s] -= zn[j][s];
   };
 
-  // l (method coefficients), tq (error constants), gamma for the current q, h, tau history
+  // l (method coefficients), tq (error constants), gamma for the current q, h, tau history.
+  // Computed in registers (loops unrolled to the maximum order, predicated on q) and written
+  // back once; the operations and their order are those of the loop form.
   auto set_bdf = [&]() __attribute__((always_inline)) {
     const int q = c.q;
     const double h = c.h;
-    double* l = c.l;
+    double tau[Q + 1], l[Q + 1];
+#pragma unroll
+    for (int i = 0; i <= Q; ++i) {
+      tau[i] = c.tau[i];
+      l[i] = 0.0;
+    }
     double alpha0 = -1.0, alpha0_hat = -1.0, xi_inv = 1.0, xistar_inv = 1.0, hsum = h;
     l[0] = 1.0;
     l[1] = 1.0;
-    for (int i = 2; i <= Q; ++i) l[i] = 0.0;
     if (q > 1) {
-      for (int j = 2; j < q; ++j) {
-        hsum += c.tau[j - 1];
-        xi_inv = h / hsum;
-        alpha0 -= 1.0 / j;
-        for (int i = j; i >= 1; --i) l[i] += l[i - 1] * xi_inv;
+#pragma unroll
+      for (int j = 2; j < Q; ++j) {
+        if (j < q) {
+          hsum += tau[j - 1];
+          xi_inv = h / hsum;
+          alpha0 -= 1.0 / j;
+#pragma unroll
+          for (int i = j; i >= 1; --i) l[i] += l[i - 1] * xi_inv;
+        }
       }
       alpha0 -= 1.0 / q;
       xistar_inv = -l[1] - alpha0;
-      hsum += c.tau[q - 1];
+      double tq1 = 0.0;
+#pragma unroll
+      for (int j = 1; j <= Q; ++j)
+        if (j == q) tq1 = tau[j - 1];
+      hsum += tq1;
       xi_inv = h / hsum;
       alpha0_hat = -l[1] - xi_inv;
-      for (int i = q; i >= 1; --i) l[i] += l[i - 1] * xistar_inv;
+#pragma unroll
+      for (int i = Q; i >= 1; --i)
+        if (i <= q) l[i] += l[i - 1] * xistar_inv;
     }
+    double lq = l[1];
+#pragma unroll
+    for (int i = 2; i <= Q; ++i)
+      if (i == q) lq = l[i];
     const double A1 = 1.0 - alpha0_hat + alpha0;
     const double A2 = 1.0 + q * A1;
-    const double lq = l[q];
     c.tq[2] = fabs(A1 / (alpha0 * A2));
     c.tq[5] = fabs(A2 * xistar_inv / (lq * xi_inv));
     if (c.qwait == 1) {
@@ -229,7 +248,11 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
       } else {
         c.tq[1] = 1.0;
       }
-      hsum += c.tau[q];
+      double tqq = 0.0;
+#pragma unroll
+      for (int j = 1; j <= Q; ++j)
+        if (j == q) tqq = tau[j];
+      hsum += tqq;
       xi_inv = h / hsum;
       double A5 = alpha0 - 1.0 / (q + 1);
       double A6 = alpha0_hat - xi_inv;
@@ -237,6 +260,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
       c.tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
     }
     c.tq[4] = 0.1 / c.tq[2];
+#pragma unroll
+    for (int i = 0; i <= Q; ++i) c.l[i] = l[i];
     c.rl1 = 1.0 / l[1];
     c.gamma = h * c.rl1;
     if (c.nst == 0) c.gammap = c.gamma;
