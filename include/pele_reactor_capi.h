@@ -13,6 +13,9 @@
  *   pr_set_typ_vals  ReactorBase::set_typ_vals_ode (Source/PeleLMeX_Utils.cpp:1472)
  *   pr_react_box     ReactorBase::react(box, rY, rYsrc, T, rE, rEsrc, FC, mask, dt, time, stream)
  *                    (Source/PeleLMeX_Reactions.cpp:101-108 and :267-274)
+ *   pr_react_batch   the per-box loop around react in advanceChemistry
+ *                    (Source/PeleLMeX_Reactions.cpp:48-108, chem-BoxArray variant :243-274):
+ *                    every box of a level, each with its own dt_react, in one launch
  *   pr_destroy       ReactorBase::close() + destructor (Source/PeleLMeX.cpp:26)
  *   pr_get_stats     no reference counterpart: integration counters of the last react call
  *                    (FC carries RHS counts per cell, as PeleLMeX's functC does, PeleLMeX.H:209)
@@ -88,6 +91,21 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
                  const pr_array4* rYsrc, const pr_array4* T, const pr_array4* rH,
                  const pr_array4* rHsrc, const pr_array4* FC, const pr_iarray4* mask,
                  double* dt_react, double* time, void* stream, int* status_out);
+/* One box of pr_react_batch: the arguments of pr_react_box for that box. */
+typedef struct {
+  int lo[3], hi[3];
+  pr_array4 rY, rYsrc, T, rH, rHsrc, FC;
+  pr_iarray4 mask;   /* mask.p may be NULL: every cell active */
+  double dt_react;
+  int* status;       /* device, one int per box cell, may be NULL */
+} pr_box;
+
+/* Integrates every box of the list over its own dt_react with the semantics of pr_react_box,
+   in one persistent launch whose work queue holds the cells of all boxes (so boxes of mixed
+   sizes and stiffness balance across the GPU). The learned map is not used on this path.
+   Returns the number of failed cells (>= 0) or a negative error code. */
+int pr_react_batch(pr_handle* h, int nbox, const pr_box* boxes, void* stream);
+
 /* Batched evaluations on device arrays stored component-major with stride ncells:
    y[i * ncells + c] (i < NUM_SPECIES + 1: rhoY then T), rYsrc[k * ncells + c], rhoh0[c],
    rhohsrc[c]. f gets the right-hand side at time t (T recomputed from rho h(t) and Y, starting

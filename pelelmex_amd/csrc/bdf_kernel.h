@@ -47,13 +47,23 @@ struct IArr {
   int bx, by, bz;
 };
 
-struct KParams {
+// One box of a react call: its index range, Array4 views, mask, per-cell status and dt_react.
+struct BoxDesc {
   int lo0, lo1, lo2, nx, ny;
   long ncells;
   Arr rY, rYsrc, T, rH, rHsrc, FC;
   IArr mask;
   int has_mask;
+  int* status;  // per cell (linear box index), nullable
   double dt;
+};
+
+struct KParams {
+  BoxDesc one;           // host side: the box of a single-box call (staged into boxes[0])
+  const BoxDesc* boxes;  // device table of the launch's boxes: queue entry q is cell
+  const long* box_start; // q - box_start[b] of box b, box_start[b] <= q < box_start[b + 1]
+  int nbox;
+  long ncells;           // queue entries (cells of all boxes)
   double rtol, atol;
   const double* atolv;  // n entries or null
   int max_order, max_steps;
@@ -61,7 +71,6 @@ struct KParams {
   int blob_bytes, neff, ninc;
   unsigned int* queue;           // work counter, zero before launch
   unsigned long long* stats;     // cells, nst, nfe, nje, nsetups, nni, netf, ncfn
-  int* status;                   // per cell (linear box index), nullable
   unsigned int* nfail;
   double* gscr;                  // per-group global scratch slots (saved Jacobian, Jacobian records)
   int clean;                     // ode.clean_init_massfrac: integrate from max(rhoY, 0), add the rest back

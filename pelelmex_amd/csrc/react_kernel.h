@@ -25,7 +25,8 @@ struct Ctl {
   double saved_tq5, saved_t, t_eval, hlb, hub, hg, rhoh0, rhohsrc;
   double tau[QMAXC + 1], tq[6], l[QMAXC + 1];
   int q, qprime, L, qwait, nscon, ncf, nef, nflag, convfail, mnewt, jcur, jbad, callSetup, count1;
-  int phase, cell, nst, nstlp, nstlj, nstloc, nfe, nje, nset, nni, netf, ncfn, guard;
+  int phase, cell, box, nst, nstlp, nstlj, nstloc, nfe, nje, nset, nni, netf, ncfn, guard;
+  double tout;
   unsigned int s_cells, s_nst, s_nfe, s_nje, s_nset, s_nni, s_netf, s_ncfn, s_fail;
 };
 constexpr int CTL_DOUBLES = (int)((sizeof(Ctl) + 7) / 8);
@@ -94,7 +95,6 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
   const int grp = threadIdx.x / G;
   Ctl& c = *reinterpret_cast<Ctl*>(setup_views<N, R, G>(S, smem, BL, grp, P.gscr));
 
-  const double tout = P.dt;
   long nwork = P.ncells;
   if (P.ncells_dev) {
     const long d = (long)*P.ncells_dev;
@@ -116,27 +116,30 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
     atol_r[s] = (P.atolv && i < n) ? P.atolv[i] : P.atol;
   }
 
-  auto cell_ijk = [&](int cl, int& i, int& j, int& k) __attribute__((always_inline)) {
+  auto box_of = [&](int b) __attribute__((always_inline)) -> const BoxDesc& { return P.boxes[b]; };
+
+  auto cell_ijk = [&](const BoxDesc& B, int cl, int& i, int& j, int& k) __attribute__((always_inline)) {
     const unsigned u = (unsigned)cl;
-    i = P.lo0 + (int)(u % (unsigned)P.nx);
-    j = P.lo1 + (int)((u / (unsigned)P.nx) % (unsigned)P.ny);
-    k = P.lo2 + (int)(u / ((unsigned)P.nx * (unsigned)P.ny));
+    i = B.lo0 + (int)(u % (unsigned)B.nx);
+    j = B.lo1 + (int)((u / (unsigned)B.nx) % (unsigned)B.ny);
+    k = B.lo2 + (int)(u / ((unsigned)B.nx * (unsigned)B.ny));
   };
 
   auto finish = [&](int code, const double (&out)[RPL]) __attribute__((always_inline)) {
+    const BoxDesc& B = box_of(c.box);
     int i, j, k;
-    cell_ijk(c.cell, i, j, k);
-    const long offY = aoff(P.rY, i, j, k), offT = aoff(P.T, i, j, k);
+    cell_ijk(B, c.cell, i, j, k);
+    const long offY = aoff(B.rY, i, j, k), offT = aoff(B.T, i, j, k);
 #pragma unroll
     for (int s = 0; s < RPL; ++s) {
       int ii = lane + G * s;
-      if (ii < N) P.rY.p[offY + (long)ii * P.rY.ns] = neg[s] < 0.0 ? out[s] + neg[s] : out[s];
-      else if (ii == N) P.T.p[offT] = out[s];
+      if (ii < N) B.rY.p[offY + (long)ii * B.rY.ns] = neg[s] < 0.0 ? out[s] + neg[s] : out[s];
+      else if (ii == N) B.T.p[offT] = out[s];
     }
     if (lane == 0) {
-      P.rH.p[aoff(P.rH, i, j, k)] = c.rhoh0 + tout * c.rhohsrc;
-      P.FC.p[aoff(P.FC, i, j, k)] = (double)c.nfe;
-      if (P.status) P.status[c.cell] = code;
+      B.rH.p[aoff(B.rH, i, j, k)] = c.rhoh0 + c.tout * c.rhohsrc;
+      B.FC.p[aoff(B.FC, i, j, k)] = (double)c.nfe;
+      if (B.status) B.status[c.cell] = code;
     }
     c.s_cells += 1u;
     c.s_nst += c.nst; c.s_nfe += c.nfe; c.s_nje += c.nje; c.s_nset += c.nset; c.s_nni += c.nni;
@@ -403,21 +406,36 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
       if (lane == 0) cl = atomicAdd(P.queue, 1u);
       cl = __shfl(cl, 0, G);
       if ((long)cl >= nwork) break;
-      if (P.cell_list) cl = (unsigned int)P.cell_list[cl];
+      int b = 0;
+      if (P.cell_list) {
+        cl = (unsigned int)P.cell_list[cl];
+      } else if (P.nbox > 1) {  // the box holding queue entry cl
+        int lo = 0, hi = P.nbox;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (P.box_start[mid] <= (long)cl) lo = mid;
+          else hi = mid;
+        }
+        b = lo;
+        cl = (unsigned int)((long)cl - P.box_start[b]);
+      }
+      const BoxDesc& B = box_of(b);
       int i, j, k;
-      cell_ijk((int)cl, i, j, k);
+      cell_ijk(B, (int)cl, i, j, k);
       c.cell = (int)cl;
-      if (P.has_mask) {
-        long mo = (long)(i - P.mask.bx) + (long)(j - P.mask.by) * P.mask.js + (long)(k - P.mask.bz) * P.mask.ks;
-        if (P.mask.p[mo] == -1) {
+      c.box = b;
+      c.tout = B.dt;
+      if (B.has_mask) {
+        long mo = (long)(i - B.mask.bx) + (long)(j - B.mask.by) * B.mask.js + (long)(k - B.mask.bz) * B.mask.ks;
+        if (B.mask.p[mo] == -1) {
           if (lane == 0) {
-            P.FC.p[aoff(P.FC, i, j, k)] = 0.0;
-            if (P.status) P.status[cl] = ST_OK;
+            B.FC.p[aoff(B.FC, i, j, k)] = 0.0;
+            if (B.status) B.status[cl] = ST_OK;
           }
           continue;
         }
       }
-      const long offY = aoff(P.rY, i, j, k), offS = aoff(P.rYsrc, i, j, k);
+      const long offY = aoff(B.rY, i, j, k), offS = aoff(B.rYsrc, i, j, k);
 #pragma unroll
       for (int s = 0; s < RPL; ++s) {
         int ii = lane + G * s;
@@ -425,22 +443,22 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
         src[s] = 0.0;
         neg[s] = 0.0;
         if (ii < N) {
-          y0 = P.rY.p[offY + (long)ii * P.rY.ns];
-          src[s] = P.rYsrc.p[offS + (long)ii * P.rYsrc.ns];
+          y0 = B.rY.p[offY + (long)ii * B.rY.ns];
+          src[s] = B.rYsrc.p[offS + (long)ii * B.rYsrc.ns];
           if (P.clean && y0 < 0.0) {  // integrate from the non-negative part
             neg[s] = y0;
             y0 = 0.0;
           }
         } else if (ii == N) {
-          y0 = P.T.p[aoff(P.T, i, j, k)];
+          y0 = B.T.p[aoff(B.T, i, j, k)];
         }
         zn[0][s] = y0;
         yv[s] = y0;
 #pragma unroll
         for (int j2 = 1; j2 <= Q; ++j2) zn[j2][s] = 0.0;
       }
-      c.rhoh0 = P.rH.p[aoff(P.rH, i, j, k)];
-      c.rhohsrc = P.rHsrc.p[aoff(P.rHsrc, i, j, k)];
+      c.rhoh0 = B.rH.p[aoff(B.rH, i, j, k)];
+      c.rhohsrc = B.rHsrc.p[aoff(B.rHsrc, i, j, k)];
       c.nfe = c.nje = c.nset = c.nni = c.netf = c.ncfn = 0;
       c.nst = c.nstlp = c.nstlj = 0;
       c.q = 1; c.L = 2; c.qwait = 2; c.qprime = 1;
@@ -475,6 +493,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
         }
       }
       hinv = gmax<G>(hinv);
+      const double tout = c.tout;
       const double tround = DBL_EPSILON * fabs(tout);
       const double hlb = 100.0 * tround;
       double hub = 0.1 * fabs(tout);
@@ -725,7 +744,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
           }
           c.etamax = 10.0;
           c.nstloc += 1;
-          const double tn = c.tn, h = c.h;
+          const double tn = c.tn, h = c.h, tout = c.tout;
           if ((tn - tout) * h >= 0.0) {
             const double sfac = (tout - tn) / h;
             double out[RPL];

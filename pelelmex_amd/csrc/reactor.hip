@@ -362,6 +362,9 @@ struct pr_handle {
   pr_stats last{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   Surrogate* surr = nullptr;
+  void* d_boxtab = nullptr;  // device box table (descriptors, then cell prefix sums)
+  void* h_boxtab = nullptr;  // pinned staging copy of it
+  size_t boxtab_bytes = 0;
 };
 
 extern "C" {
@@ -489,6 +492,30 @@ static int bdf_launch(pr_handle* h, prk::KParams& P, long work_max, hipStream_t 
   return 0;
 }
 
+// Copies box descriptors and their cell prefix sums to the device table (through pinned
+// staging memory, reused after the call's final stream synchronization).
+static int stage_boxes(pr_handle* h, const std::vector<prk::BoxDesc>& d, const std::vector<long>& start,
+                       hipStream_t st, const prk::BoxDesc** dd, const long** ds) {
+  const size_t nd = d.size() * sizeof(prk::BoxDesc), ns = start.size() * sizeof(long);
+  const size_t off = (nd + 255) / 256 * 256, bytes = off + ns;
+  if (bytes > h->boxtab_bytes) {
+    if (h->d_boxtab) (void)hipFree(h->d_boxtab);
+    if (h->h_boxtab) (void)hipHostFree(h->h_boxtab);
+    h->d_boxtab = h->h_boxtab = nullptr;
+    h->boxtab_bytes = 0;
+    HIPCHK(hipMalloc(&h->d_boxtab, bytes));
+    HIPCHK(hipHostMalloc(&h->h_boxtab, bytes, hipHostMallocDefault));
+    h->boxtab_bytes = bytes;
+  }
+  auto* hb = static_cast<unsigned char*>(h->h_boxtab);
+  std::memcpy(hb, d.data(), nd);
+  std::memcpy(hb + off, start.data(), ns);
+  HIPCHK(hipMemcpyAsync(h->d_boxtab, h->h_boxtab, bytes, hipMemcpyHostToDevice, st));
+  *dd = static_cast<const prk::BoxDesc*>(h->d_boxtab);
+  *ds = reinterpret_cast<const long*>(static_cast<unsigned char*>(h->d_boxtab) + off);
+  return 0;
+}
+
 static int surr_alloc(Surrogate* S, void** p, size_t bytes) {
   HIPCHK(hipMalloc(p, bytes));
   S->owned.push_back(*p);
@@ -594,28 +621,29 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   HIPCHK(hipSetDevice(h->device));
   hipStream_t st = (hipStream_t)stream;
   prk::KParams P{};
-  P.lo0 = lo[0];
-  P.lo1 = lo[1];
-  P.lo2 = lo[2];
-  P.nx = (int)nx;
-  P.ny = (int)ny;
+  P.one.lo0 = lo[0];
+  P.one.lo1 = lo[1];
+  P.one.lo2 = lo[2];
+  P.one.nx = (int)nx;
+  P.one.ny = (int)ny;
   P.ncells = ncells;
-  P.rY = to_arr(rY);
-  P.rYsrc = to_arr(rYsrc);
-  P.T = to_arr(T);
-  P.rH = to_arr(rH);
-  P.rHsrc = to_arr(rHsrc);
-  P.FC = to_arr(FC);
-  P.has_mask = mask && mask->p;
-  if (P.has_mask) {
-    P.mask.p = mask->p;
-    P.mask.js = mask->jstride;
-    P.mask.ks = mask->kstride;
-    P.mask.bx = mask->begin[0];
-    P.mask.by = mask->begin[1];
-    P.mask.bz = mask->begin[2];
+  P.one.ncells = ncells;
+  P.one.rY = to_arr(rY);
+  P.one.rYsrc = to_arr(rYsrc);
+  P.one.T = to_arr(T);
+  P.one.rH = to_arr(rH);
+  P.one.rHsrc = to_arr(rHsrc);
+  P.one.FC = to_arr(FC);
+  P.one.has_mask = mask && mask->p;
+  if (P.one.has_mask) {
+    P.one.mask.p = mask->p;
+    P.one.mask.js = mask->jstride;
+    P.one.mask.ks = mask->kstride;
+    P.one.mask.bx = mask->begin[0];
+    P.one.mask.by = mask->begin[1];
+    P.one.mask.bz = mask->begin[2];
   }
-  P.dt = dt;
+  P.one.dt = dt;
   P.rtol = h->keys.rtol;
   P.atol = h->keys.atol;
   P.atolv = h->use_typ ? h->d_atolv : nullptr;
@@ -630,7 +658,7 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   P.queue = reinterpret_cast<unsigned int*>(base);
   P.nfail = reinterpret_cast<unsigned int*>(base + 4);
   P.stats = reinterpret_cast<unsigned long long*>(base + 64);
-  P.status = status_out;
+  P.one.status = status_out;
   h->last = pr_stats{};
 
   Surrogate* S = (h->surr && h->keys.surr_enable) ? h->surr : nullptr;
@@ -652,13 +680,13 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
     F.nx = (int)nx;
     F.ny = (int)ny;
     F.ncells = ncells;
-    F.rY = P.rY;
-    F.rYsrc = P.rYsrc;
-    F.T = P.T;
-    F.rH = P.rH;
-    F.rHsrc = P.rHsrc;
-    F.mask = P.mask;
-    F.has_mask = P.has_mask;
+    F.rY = P.one.rY;
+    F.rYsrc = P.one.rYsrc;
+    F.T = P.one.T;
+    F.rH = P.one.rH;
+    F.rHsrc = P.one.rHsrc;
+    F.mask = P.one.mask;
+    F.has_mask = P.one.has_mask;
     F.clean = P.clean;
     F.dt = dt;
     F.xmu = S->xmu;
@@ -680,14 +708,14 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
     Q.nx = (int)nx;
     Q.ny = (int)ny;
     Q.ncells = ncells;
-    Q.rY = P.rY;
-    Q.rYsrc = P.rYsrc;
-    Q.T = P.T;
-    Q.rH = P.rH;
-    Q.rHsrc = P.rHsrc;
-    Q.FC = P.FC;
-    Q.mask = P.mask;
-    Q.has_mask = P.has_mask;
+    Q.rY = P.one.rY;
+    Q.rYsrc = P.one.rYsrc;
+    Q.T = P.one.T;
+    Q.rH = P.one.rH;
+    Q.rHsrc = P.one.rHsrc;
+    Q.FC = P.one.FC;
+    Q.mask = P.one.mask;
+    Q.has_mask = P.one.has_mask;
     Q.clean = P.clean;
     Q.status = status_out;
     Q.dt = dt;
@@ -721,15 +749,12 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
     P.ncells_dev = &S->cnt[0];
   }
   HIPCHK(hipEventRecord(h->ev1, st));
-  {
-    int rc = bdf_launch(h, P, ncells, st);
-    if (rc) return rc;
-  }
-  HIPCHK(hipEventRecord(h->ev2, st));
+  // box table: [0] this box; [1] the audit batch (a 1D box of acap cells) when there is one
   const unsigned int acap = S ? S->aud_cap : 0u;
+  std::vector<prk::BoxDesc> descs(1, P.one);
+  std::vector<long> starts = {0L, ncells};
+  prk::BoxDesc adesc = P.one;
   if (S && acap > 0) {
-    // audit: the BDF re-solves the sampled accepted cells from their inputs, as a 1D batch of
-    // stride acap, with its own queue and counters (second half of d_counters)
     const long c = acap;
     double* a_rY = S->aud_box;
     double* a_src = a_rY + (long)N * c;
@@ -737,15 +762,6 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
     double* a_rH = a_T + c;
     double* a_rHsrc = a_rH + c;
     double* a_FC = a_rHsrc + c;
-    const unsigned grid = (unsigned)((c + 127) / 128);
-    hipLaunchKernelGGL(prk::audit_unpack_kernel, dim3(grid), dim3(128), 0, st, N, &S->cnt[1], acap, S->aud_in, a_rY,
-                       a_src, a_T, a_rH, a_rHsrc, a_FC);
-    HIPCHK(hipGetLastError());
-    prk::KParams A = P;
-    A.lo0 = A.lo1 = A.lo2 = 0;
-    A.nx = (int)c;
-    A.ny = 1;
-    A.ncells = c;
     auto arr1 = [&](double* p, long ns) {
       prk::Arr r;
       r.p = p;
@@ -755,14 +771,50 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
       r.bx = r.by = r.bz = 0;
       return r;
     };
-    A.rY = arr1(a_rY, c);
-    A.rYsrc = arr1(a_src, c);
-    A.T = arr1(a_T, 0);
-    A.rH = arr1(a_rH, 0);
-    A.rHsrc = arr1(a_rHsrc, 0);
-    A.FC = arr1(a_FC, 0);
-    A.has_mask = 0;
-    A.status = nullptr;
+    adesc.lo0 = adesc.lo1 = adesc.lo2 = 0;
+    adesc.nx = (int)c;
+    adesc.ny = 1;
+    adesc.ncells = c;
+    adesc.rY = arr1(a_rY, c);
+    adesc.rYsrc = arr1(a_src, c);
+    adesc.T = arr1(a_T, 0);
+    adesc.rH = arr1(a_rH, 0);
+    adesc.rHsrc = arr1(a_rHsrc, 0);
+    adesc.FC = arr1(a_FC, 0);
+    adesc.has_mask = 0;
+    adesc.status = nullptr;
+    descs.push_back(adesc);
+    starts.push_back(0L);
+    starts.push_back(c);
+  }
+  const prk::BoxDesc* dboxes = nullptr;
+  const long* dstart = nullptr;
+  {
+    int rc = stage_boxes(h, descs, starts, st, &dboxes, &dstart);
+    if (rc) return rc;
+  }
+  P.boxes = dboxes;
+  P.box_start = dstart;
+  P.nbox = 1;
+  {
+    int rc = bdf_launch(h, P, ncells, st);
+    if (rc) return rc;
+  }
+  HIPCHK(hipEventRecord(h->ev2, st));
+  if (S && acap > 0) {
+    // audit: the BDF re-solves the sampled accepted cells from their inputs, as the 1D box
+    // [1] of the table, with its own queue and counters (second half of d_counters)
+    const long c = acap;
+    const unsigned grid = (unsigned)((c + 127) / 128);
+    hipLaunchKernelGGL(prk::audit_unpack_kernel, dim3(grid), dim3(128), 0, st, N, &S->cnt[1], acap, S->aud_in,
+                       adesc.rY.p, adesc.rYsrc.p, adesc.T.p, adesc.rH.p, adesc.rHsrc.p, adesc.FC.p);
+    HIPCHK(hipGetLastError());
+    prk::KParams A = P;
+    A.one = adesc;
+    A.ncells = c;
+    A.boxes = dboxes + 1;
+    A.box_start = dstart + 2;
+    A.nbox = 1;
     A.cell_list = nullptr;
     A.ncells_dev = &S->cnt[1];
     A.queue = reinterpret_cast<unsigned int*>(base + 128);
@@ -772,8 +824,8 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
       int rc = bdf_launch(h, A, c, st);
       if (rc) return rc;
     }
-    hipLaunchKernelGGL(prk::audit_err_kernel, dim3(grid), dim3(128), 0, st, N, &S->cnt[1], acap, a_rY, a_T,
-                       S->aud_out, S->aud_err);
+    hipLaunchKernelGGL(prk::audit_err_kernel, dim3(grid), dim3(128), 0, st, N, &S->cnt[1], acap, adesc.rY.p,
+                       adesc.T.p, S->aud_out, S->aud_err);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(h->ev3, st));
@@ -821,6 +873,106 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
     h->last.audit_failed = afail;
   }
   if (time) *time += dt;
+  return (int)nfail;
+}
+
+int pr_react_batch(pr_handle* h, int nbox, const pr_box* boxes, void* stream) {
+  if (!h || nbox < 1 || !boxes) return fail(PR_ERR_ARG, "bad argument");
+  const int N = h->mech->nsp;
+  std::vector<prk::BoxDesc> descs((size_t)nbox);
+  std::vector<long> starts((size_t)nbox + 1, 0L);
+  for (int b = 0; b < nbox; ++b) {
+    const pr_box& x = boxes[b];
+    if (!arr_ok(&x.rY, N) || !arr_ok(&x.rYsrc, N) || !arr_ok(&x.T, 1) || !arr_ok(&x.rH, 1) ||
+        !arr_ok(&x.rHsrc, 1) || !arr_ok(&x.FC, 1))
+      return fail(PR_ERR_ARG, "box " + std::to_string(b) + ": missing array");
+    if ((x.rY.ncomp && x.rY.ncomp < N) || (x.rYsrc.ncomp && x.rYsrc.ncomp < N))
+      return fail(PR_ERR_ARG, "box " + std::to_string(b) + ": rY / rYsrc need NUM_SPECIES components");
+    const long nx = (long)x.hi[0] - x.lo[0] + 1, ny = (long)x.hi[1] - x.lo[1] + 1, nz = (long)x.hi[2] - x.lo[2] + 1;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return fail(PR_ERR_ARG, "box " + std::to_string(b) + ": empty box");
+    if (!(x.dt_react > 0.0) || !std::isfinite(x.dt_react))
+      return fail(PR_ERR_ARG, "box " + std::to_string(b) + ": dt_react must be positive and finite");
+    prk::BoxDesc& d = descs[(size_t)b];
+    d.lo0 = x.lo[0];
+    d.lo1 = x.lo[1];
+    d.lo2 = x.lo[2];
+    d.nx = (int)nx;
+    d.ny = (int)ny;
+    d.ncells = nx * ny * nz;
+    d.rY = to_arr(&x.rY);
+    d.rYsrc = to_arr(&x.rYsrc);
+    d.T = to_arr(&x.T);
+    d.rH = to_arr(&x.rH);
+    d.rHsrc = to_arr(&x.rHsrc);
+    d.FC = to_arr(&x.FC);
+    d.has_mask = x.mask.p != nullptr;
+    if (d.has_mask) {
+      d.mask.p = x.mask.p;
+      d.mask.js = x.mask.jstride;
+      d.mask.ks = x.mask.kstride;
+      d.mask.bx = x.mask.begin[0];
+      d.mask.by = x.mask.begin[1];
+      d.mask.bz = x.mask.begin[2];
+    }
+    d.status = x.status;
+    d.dt = x.dt_react;
+    starts[(size_t)b + 1] = starts[(size_t)b] + d.ncells;
+  }
+  const long total = starts.back();
+  if (total > 0x7fffffffL) return fail(PR_ERR_ARG, "too many cells");
+  HIPCHK(hipSetDevice(h->device));
+  hipStream_t st = (hipStream_t)stream;
+  prk::KParams P{};
+  P.one = descs[0];
+  P.ncells = total;
+  P.rtol = h->keys.rtol;
+  P.atol = h->keys.atol;
+  P.atolv = h->use_typ ? h->d_atolv : nullptr;
+  P.max_order = h->keys.max_order;
+  P.max_steps = h->keys.max_substeps;
+  P.clean = h->keys.clean_init_massfrac;
+  P.blob = h->d_blob;
+  P.blob_bytes = h->blob_bytes;
+  P.neff = h->mech->neff;
+  P.ninc = h->mech->ninc;
+  auto* base = static_cast<unsigned char*>(h->d_counters);
+  P.queue = reinterpret_cast<unsigned int*>(base);
+  P.nfail = reinterpret_cast<unsigned int*>(base + 4);
+  P.stats = reinterpret_cast<unsigned long long*>(base + 64);
+  h->last = pr_stats{};
+  {
+    const prk::BoxDesc* dboxes = nullptr;
+    const long* dstart = nullptr;
+    int rc = stage_boxes(h, descs, starts, st, &dboxes, &dstart);
+    if (rc) return rc;
+    P.boxes = dboxes;
+    P.box_start = dstart;
+    P.nbox = nbox;
+  }
+  HIPCHK(hipEventRecord(h->ev0, st));
+  {
+    int rc = bdf_launch(h, P, total, st);
+    if (rc) return rc;
+  }
+  HIPCHK(hipEventRecord(h->ev1, st));
+  unsigned long long hs[8];
+  unsigned int nfail = 0;
+  HIPCHK(hipMemcpyAsync(hs, P.stats, sizeof(hs), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&nfail, P.nfail, sizeof(nfail), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last.cells = (int64_t)hs[0];
+  h->last.nst = (int64_t)hs[1];
+  h->last.nfe = (int64_t)hs[2];
+  h->last.nje = (int64_t)hs[3];
+  h->last.nsetups = (int64_t)hs[4];
+  h->last.nni = (int64_t)hs[5];
+  h->last.netf = (int64_t)hs[6];
+  h->last.ncfn = (int64_t)hs[7];
+  h->last.failed = nfail;
+  h->last.kernel_ms = ms;
+  h->last.total_ms = ms;
   return (int)nfail;
 }
 
@@ -1127,6 +1279,8 @@ void pr_destroy(pr_handle* h) {
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->ev2) (void)hipEventDestroy(h->ev2);
   if (h->ev3) (void)hipEventDestroy(h->ev3);
+  if (h->d_boxtab) (void)hipFree(h->d_boxtab);
+  if (h->h_boxtab) (void)hipHostFree(h->h_boxtab);
   if (h->surr) {
     for (void* q : h->surr->owned) (void)hipFree(q);
     delete h->surr;

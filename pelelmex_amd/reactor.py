@@ -22,7 +22,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpelelmex_amd.so")
 _LIB = None
 
-EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_rhs_batch", "pr_jac_batch",
+EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_react_batch", "pr_rhs_batch", "pr_jac_batch",
            "pr_reaction_rate_box", "pr_surrogate_load", "pr_surrogate_features", "pr_mlp_gemm",
            "pr_mlp_gelu_backward", "pr_mlp_column_sum", "pr_mlp_mse", "pr_mlp_adam", "pr_mlp_to_bf16",
            "pr_get_stats", "pr_num_species", "pr_destroy", "pr_last_error"]
@@ -36,6 +36,12 @@ class Array4(C.Structure):
 class IArray4(C.Structure):
     _fields_ = [("p", C.c_void_p), ("jstride", C.c_int64), ("kstride", C.c_int64),
                 ("begin", C.c_int * 3)]
+
+
+class Box(C.Structure):
+    _fields_ = [("lo", C.c_int * 3), ("hi", C.c_int * 3), ("rY", Array4), ("rYsrc", Array4), ("T", Array4),
+                ("rH", Array4), ("rHsrc", Array4), ("FC", Array4), ("mask", IArray4), ("dt_react", C.c_double),
+                ("status", C.c_void_p)]
 
 
 class Stats(C.Structure):
@@ -67,6 +73,8 @@ def lib():
                                    C.POINTER(Array4), C.POINTER(IArray4), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
         L.pr_react_box.restype = C.c_int
+        L.pr_react_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(Box), C.c_void_p]
+        L.pr_react_batch.restype = C.c_int
         L.pr_rhs_batch.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_double, C.c_void_p, C.c_void_p]
         L.pr_jac_batch.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -167,6 +175,25 @@ class Reactor:
         if rc < 0:
             raise ReactorError(f"pr_react_box failed ({rc}): {lib().pr_last_error().decode()}")
         return rc, dt.value, tm.value
+
+    def react_batch(self, boxes, stream=None):
+        """Several boxes in one launch. boxes: list of dicts with device tensors rY, rYsrc, T, rH,
+        rHsrc, FC (shapes as for react), optional mask and status, and dt. Returns n_failed."""
+        import torch
+        arr = (Box * len(boxes))()
+        for b, x in enumerate(boxes):
+            nz, ny, nx = x["T"].shape
+            m = x.get("mask")
+            arr[b] = Box((C.c_int * 3)(0, 0, 0), (C.c_int * 3)(nx - 1, ny - 1, nz - 1), _array4(x["rY"], self.nsp),
+                         _array4(x["rYsrc"], self.nsp), _array4(x["T"], 1), _array4(x["rH"], 1),
+                         _array4(x["rHsrc"], 1), _array4(x["FC"], 1),
+                         IArray4(m.data_ptr() if m is not None else None, nx, nx * ny, (C.c_int * 3)(0, 0, 0)),
+                         float(x["dt"]), x["status"].data_ptr() if x.get("status") is not None else None)
+        st = stream if stream is not None else torch.cuda.current_stream()
+        rc = lib().pr_react_batch(self._h, len(boxes), arr, C.c_void_p(st.cuda_stream))
+        if rc < 0:
+            raise ReactorError(f"pr_react_batch failed ({rc}): {lib().pr_last_error().decode()}")
+        return rc
 
     def rhs_batch(self, y, rYsrc, rhoh0, rhohsrc, t=0.0, stream=None):
         """Right-hand side for a batch: y [n, ncells], rYsrc [N, ncells] (device tensors)."""
