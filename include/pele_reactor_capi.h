@@ -105,6 +105,15 @@ typedef struct {
    sizes and stiffness balance across the GPU). The learned map is not used on this path.
    Returns the number of failed cells (>= 0) or a negative error code. */
 int pr_react_batch(pr_handle* h, int nbox, const pr_box* boxes, void* stream);
+/* pr_react_batch in two halves, for callers that capture the work in a hipGraph: _enqueue only
+   enqueues on `stream` (a copy of the box table from the handle's pinned staging memory, the
+   counter reset kernel and the integration kernel: no synchronization, no memset node, and no
+   allocation once a first call of the same or larger size has run); pr_collect waits for the
+   stream and returns the failed-cell count of the last integration, filling pr_get_stats. A
+   captured batch replays the box table staged at capture; other react calls on the same handle
+   restage it. */
+int pr_react_batch_enqueue(pr_handle* h, int nbox, const pr_box* boxes, void* stream);
+int pr_collect(pr_handle* h, void* stream);
 
 /* Batched evaluations on device arrays stored component-major with stride ncells:
    y[i * ncells + c] (i < NUM_SPECIES + 1: rhoY then T), rYsrc[k * ncells + c], rhoh0[c],

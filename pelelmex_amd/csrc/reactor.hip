@@ -876,7 +876,7 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   return (int)nfail;
 }
 
-int pr_react_batch(pr_handle* h, int nbox, const pr_box* boxes, void* stream) {
+int pr_react_batch_enqueue(pr_handle* h, int nbox, const pr_box* boxes, void* stream) {
   if (!h || nbox < 1 || !boxes) return fail(PR_ERR_ARG, "bad argument");
   const int N = h->mech->nsp;
   std::vector<prk::BoxDesc> descs((size_t)nbox);
@@ -939,7 +939,6 @@ int pr_react_batch(pr_handle* h, int nbox, const pr_box* boxes, void* stream) {
   P.queue = reinterpret_cast<unsigned int*>(base);
   P.nfail = reinterpret_cast<unsigned int*>(base + 4);
   P.stats = reinterpret_cast<unsigned long long*>(base + 64);
-  h->last = pr_stats{};
   {
     const prk::BoxDesc* dboxes = nullptr;
     const long* dstart = nullptr;
@@ -949,19 +948,27 @@ int pr_react_batch(pr_handle* h, int nbox, const pr_box* boxes, void* stream) {
     P.box_start = dstart;
     P.nbox = nbox;
   }
-  HIPCHK(hipEventRecord(h->ev0, st));
   {
     int rc = bdf_launch(h, P, total, st);
     if (rc) return rc;
   }
-  HIPCHK(hipEventRecord(h->ev1, st));
+  return 0;
+}
+
+int pr_collect(pr_handle* h, void* stream) {
+  if (!h) return fail(PR_ERR_ARG, "NULL argument");
+  HIPCHK(hipSetDevice(h->device));
+  hipStream_t st = (hipStream_t)stream;
+  auto* base = static_cast<unsigned char*>(h->d_counters);
   unsigned long long hs[8];
   unsigned int nfail = 0;
-  HIPCHK(hipMemcpyAsync(hs, P.stats, sizeof(hs), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(&nfail, P.nfail, sizeof(nfail), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(hs, base + 64, sizeof(hs), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&nfail, base + 4, sizeof(nfail), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  const int G = h->last.lanes_per_cell, blocks = h->last.blocks;
+  h->last = pr_stats{};
+  h->last.lanes_per_cell = G;
+  h->last.blocks = blocks;
   h->last.cells = (int64_t)hs[0];
   h->last.nst = (int64_t)hs[1];
   h->last.nfe = (int64_t)hs[2];
@@ -971,9 +978,22 @@ int pr_react_batch(pr_handle* h, int nbox, const pr_box* boxes, void* stream) {
   h->last.netf = (int64_t)hs[6];
   h->last.ncfn = (int64_t)hs[7];
   h->last.failed = nfail;
+  return (int)nfail;
+}
+
+int pr_react_batch(pr_handle* h, int nbox, const pr_box* boxes, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (h) HIPCHK(hipEventRecord(h->ev0, st));
+  int rc = pr_react_batch_enqueue(h, nbox, boxes, stream);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(h->ev1, st));
+  rc = pr_collect(h, stream);
+  if (rc < 0) return rc;
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   h->last.kernel_ms = ms;
   h->last.total_ms = ms;
-  return (int)nfail;
+  return rc;
 }
 
 static int eval_batch(pr_handle* h, int mode, int64_t ncells, const double* y, const double* rYsrc,

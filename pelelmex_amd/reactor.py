@@ -22,7 +22,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpelelmex_amd.so")
 _LIB = None
 
-EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_react_batch", "pr_rhs_batch", "pr_jac_batch",
+EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_react_batch", "pr_react_batch_enqueue", "pr_collect", "pr_rhs_batch", "pr_jac_batch",
            "pr_reaction_rate_box", "pr_surrogate_load", "pr_surrogate_features", "pr_mlp_gemm",
            "pr_mlp_gelu_backward", "pr_mlp_column_sum", "pr_mlp_mse", "pr_mlp_adam", "pr_mlp_to_bf16",
            "pr_get_stats", "pr_num_species", "pr_destroy", "pr_last_error"]
@@ -75,6 +75,8 @@ def lib():
         L.pr_react_box.restype = C.c_int
         L.pr_react_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(Box), C.c_void_p]
         L.pr_react_batch.restype = C.c_int
+        L.pr_react_batch_enqueue.argtypes = [C.c_void_p, C.c_int, C.POINTER(Box), C.c_void_p]
+        L.pr_collect.argtypes = [C.c_void_p, C.c_void_p]
         L.pr_rhs_batch.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_double, C.c_void_p, C.c_void_p]
         L.pr_jac_batch.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -176,9 +178,10 @@ class Reactor:
             raise ReactorError(f"pr_react_box failed ({rc}): {lib().pr_last_error().decode()}")
         return rc, dt.value, tm.value
 
-    def react_batch(self, boxes, stream=None):
+    def react_batch(self, boxes, stream=None, enqueue_only=False):
         """Several boxes in one launch. boxes: list of dicts with device tensors rY, rYsrc, T, rH,
-        rHsrc, FC (shapes as for react), optional mask and status, and dt. Returns n_failed."""
+        rHsrc, FC (shapes as for react), optional mask and status, and dt. Returns n_failed, or
+        0 after only enqueueing the work when enqueue_only (then collect() waits for it)."""
         import torch
         arr = (Box * len(boxes))()
         for b, x in enumerate(boxes):
@@ -190,9 +193,19 @@ class Reactor:
                          IArray4(m.data_ptr() if m is not None else None, nx, nx * ny, (C.c_int * 3)(0, 0, 0)),
                          float(x["dt"]), x["status"].data_ptr() if x.get("status") is not None else None)
         st = stream if stream is not None else torch.cuda.current_stream()
-        rc = lib().pr_react_batch(self._h, len(boxes), arr, C.c_void_p(st.cuda_stream))
+        fn = lib().pr_react_batch_enqueue if enqueue_only else lib().pr_react_batch
+        rc = fn(self._h, len(boxes), arr, C.c_void_p(st.cuda_stream))
         if rc < 0:
             raise ReactorError(f"pr_react_batch failed ({rc}): {lib().pr_last_error().decode()}")
+        return rc
+
+    def collect(self, stream=None):
+        """pr_collect: waits for the stream; failed cells of the last enqueued integration."""
+        import torch
+        st = stream if stream is not None else torch.cuda.current_stream()
+        rc = lib().pr_collect(self._h, C.c_void_p(st.cuda_stream))
+        if rc < 0:
+            raise ReactorError(f"pr_collect failed ({rc}): {lib().pr_last_error().decode()}")
         return rc
 
     def rhs_batch(self, y, rYsrc, rhoh0, rhohsrc, t=0.0, stream=None):
