@@ -88,7 +88,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--lanes", type=int, default=32, help="hipbdf.lanes_per_cell")
+    ap.add_argument("--lanes", type=int, default=32, help="hipbdf.lanes_per_cell (0: autotune)")
     ap.add_argument("--max-order", type=int, default=4)
     ap.add_argument("--cpu-sample", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true")
@@ -113,8 +113,11 @@ def main():
     t = box_tensors(box, dev)
     pristine = {k: t[k].clone() for k in ("rY", "T", "rH")}
     keys = {"ode.rtol": 1e-10, "ode.atol": 1e-10, "cvode.max_order": args.max_order,
-            "cvode.max_substeps": 10000, "cvode.solve_type": "denseAJ_direct",
-            "hipbdf.lanes_per_cell": args.lanes}
+            "cvode.max_substeps": 10000, "cvode.solve_type": "denseAJ_direct"}
+    if args.lanes:
+        keys["hipbdf.lanes_per_cell"] = args.lanes
+    else:
+        keys["hipbdf.autotune"] = 1
 
     def measure(r):
         """W untimed + K timed react calls on the pristine box; (ms per call, last stats, state)."""

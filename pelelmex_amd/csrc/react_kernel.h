@@ -928,6 +928,27 @@ __global__ void __launch_bounds__(BLOCK) rate_kernel(RateParams P) {
   }
 }
 
+// S evenly spaced cells of a box, component-major with stride S: rhoY (N), sources (N), T, rho h,
+// its source, FC (zero).
+__global__ void gather_sample_kernel(BoxDesc B, int N, long S, double* out) {
+  const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= S) return;
+  const long cl = a * (B.ncells / S);
+  const unsigned u = (unsigned)cl;
+  const int i = B.lo0 + (int)(u % (unsigned)B.nx);
+  const int j = B.lo1 + (int)((u / (unsigned)B.nx) % (unsigned)B.ny);
+  const int k = B.lo2 + (int)(u / ((unsigned)B.nx * (unsigned)B.ny));
+  const long offY = aoff(B.rY, i, j, k), offS = aoff(B.rYsrc, i, j, k);
+  for (int n = 0; n < N; ++n) {
+    out[(long)n * S + a] = B.rY.p[offY + (long)n * B.rY.ns];
+    out[(long)(N + n) * S + a] = B.rYsrc.p[offS + (long)n * B.rYsrc.ns];
+  }
+  out[(2L * N) * S + a] = B.T.p[aoff(B.T, i, j, k)];
+  out[(2L * N + 1) * S + a] = B.rH.p[aoff(B.rH, i, j, k)];
+  out[(2L * N + 2) * S + a] = B.rHsrc.p[aoff(B.rHsrc, i, j, k)];
+  out[(2L * N + 3) * S + a] = 0.0;
+}
+
 __global__ void zero_counters(unsigned int* queue, unsigned long long* stats, unsigned int* nfail) {
   int t = threadIdx.x;
   if (t < 8) stats[t] = 0ull;

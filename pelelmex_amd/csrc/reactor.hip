@@ -227,7 +227,7 @@ struct Keys {
   int max_order = 2, max_substeps = 10000;
   std::string solve_type = "GMRES";
   int clean_init_massfrac = 0, analytical_jacobian = 1, verbose = 0;
-  int lanes_per_cell = 0, block = 0;
+  int lanes_per_cell = 0, block = 0, autotune = 0;
   // learned map in front of the BDF (used once weights are loaded with pr_surrogate_load)
   int surr_enable = 1;
   double surr_trust_rtol = 1e-3, surr_trust_atol = 1e-9, surr_elem_tol = 0.05, surr_audit_frac = 0.01;
@@ -293,6 +293,9 @@ int parse_keys(const char* text, Keys& K) {
     } else if (k == "hipbdf.lanes_per_cell") {
       if (!num(d)) return fail(PR_ERR_KEY, "bad hipbdf.lanes_per_cell");
       K.lanes_per_cell = (int)d;
+    } else if (k == "hipbdf.autotune") {
+      if (!num(d) || (d != 0 && d != 1)) return fail(PR_ERR_KEY, "hipbdf.autotune must be 0 or 1");
+      K.autotune = (int)d;
     } else if (k == "hipbdf.block") {
       if (!num(d) || ((int)d % 64) != 0 || d < 64 || d > 256) return fail(PR_ERR_KEY, "hipbdf.block must be 64, 128, 192 or 256");
       K.block = (int)d;
@@ -362,6 +365,7 @@ struct pr_handle {
   pr_stats last{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   Surrogate* surr = nullptr;
+  bool tuned = false;        // hipbdf.autotune has chosen var / block
   void* d_boxtab = nullptr;  // device box table (descriptors, then cell prefix sums)
   void* h_boxtab = nullptr;  // pinned staging copy of it
   size_t boxtab_bytes = 0;
@@ -602,6 +606,99 @@ static int mlp_forward(pr_handle* h, long ncells, hipStream_t st) {
   return 0;
 }
 
+// hipbdf.autotune: on the first react call, every lanes-per-cell variant of the mechanism and
+// block sizes of 128 and 256 threads integrate the same evenly spaced sample (up to 65536
+// cells) of the box (a copy:
+// the box itself is untouched); the fastest per cell is kept for the handle. All variants give
+// identical results, so the choice only affects speed.
+static int autotune(pr_handle* h, const prk::KParams& Pbase, const prk::BoxDesc& B, hipStream_t st) {
+  if (!h->keys.autotune || h->tuned) return 0;
+  h->tuned = true;
+  const int N = h->mech->nsp;
+  // the sample must be many times the cells resident on the GPU at once, or the launch tail
+  // (not the steady state) decides the ranking
+  const long S = B.ncells < 65536 ? B.ncells : 65536;
+  if (S < 4096) return 0;  // too small to tell variants apart
+  double* buf = nullptr;
+  const size_t per = (size_t)(2 * N + 4);
+  HIPCHK(hipMalloc(&buf, sizeof(double) * S * per * 2));
+  double* pristine = buf;
+  double* work = buf + S * per;
+  hipLaunchKernelGGL(prk::gather_sample_kernel, dim3((unsigned)((S + 127) / 128)), dim3(128), 0, st, B, N, S,
+                     pristine);
+  hipError_t e = hipGetLastError();
+  const Variant* best_v = h->var;
+  int best_b = h->block;
+  float best_ms = 1e30f;
+  prk::BoxDesc sd{};
+  sd.nx = (int)S;
+  sd.ny = 1;
+  sd.ncells = S;
+  auto arr1 = [&](double* p, long ns) {
+    prk::Arr r;
+    r.p = p;
+    r.js = S;
+    r.ks = S;
+    r.ns = ns;
+    r.bx = r.by = r.bz = 0;
+    return r;
+  };
+  sd.rY = arr1(work, S);
+  sd.rYsrc = arr1(work + (long)N * S, S);
+  sd.T = arr1(work + 2L * N * S, 0);
+  sd.rH = arr1(work + (2L * N + 1) * S, 0);
+  sd.rHsrc = arr1(work + (2L * N + 2) * S, 0);
+  sd.FC = arr1(work + (2L * N + 3) * S, 0);
+  sd.dt = B.dt;
+  std::vector<prk::BoxDesc> descs(1, sd);
+  std::vector<long> starts = {0L, S};
+  const Variant* v0 = h->var;
+  const int b0 = h->block;
+  for (const Variant& v : kVariants) {
+    if (e != hipSuccess) break;
+    if (std::strcmp(v.mech, h->mech->name) != 0) continue;
+    for (int blk : {128, 256}) {
+      h->var = &v;
+      h->block = blk;
+      prk::KParams P = Pbase;
+      const prk::BoxDesc* db = nullptr;
+      const long* dsx = nullptr;
+      if ((e = hipMemcpyAsync(work, pristine, sizeof(double) * S * per, hipMemcpyDeviceToDevice, st)) != hipSuccess) break;
+      int rc = stage_boxes(h, descs, starts, st, &db, &dsx);
+      if (rc) {
+        e = hipErrorUnknown;
+        break;
+      }
+      P.boxes = db;
+      P.box_start = dsx;
+      P.nbox = 1;
+      P.ncells = S;
+      P.cell_list = nullptr;
+      P.ncells_dev = nullptr;
+      if ((e = hipEventRecord(h->ev2, st)) != hipSuccess) break;
+      rc = bdf_launch(h, P, S, st);
+      if (rc) {
+        e = hipErrorUnknown;
+        break;
+      }
+      if ((e = hipEventRecord(h->ev3, st)) != hipSuccess || (e = hipStreamSynchronize(st)) != hipSuccess) break;
+      float ms = 0.f;
+      if ((e = hipEventElapsedTime(&ms, h->ev2, h->ev3)) != hipSuccess) break;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_v = &v;
+        best_b = blk;
+      }
+    }
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(buf);
+  h->var = (e == hipSuccess) ? best_v : v0;
+  h->block = (e == hipSuccess) ? best_b : b0;
+  if (e != hipSuccess) return fail(PR_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+  return 0;
+}
+
 int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY,
                  const pr_array4* rYsrc, const pr_array4* T, const pr_array4* rH,
                  const pr_array4* rHsrc, const pr_array4* FC, const pr_iarray4* mask,
@@ -661,6 +758,10 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   P.one.status = status_out;
   h->last = pr_stats{};
 
+  {
+    int rc = autotune(h, P, P.one, st);
+    if (rc) return rc;
+  }
   Surrogate* S = (h->surr && h->keys.surr_enable) ? h->surr : nullptr;
   HIPCHK(hipEventRecord(h->ev0, st));
   if (S) {
@@ -939,6 +1040,13 @@ int pr_react_batch_enqueue(pr_handle* h, int nbox, const pr_box* boxes, void* st
   P.queue = reinterpret_cast<unsigned int*>(base);
   P.nfail = reinterpret_cast<unsigned int*>(base + 4);
   P.stats = reinterpret_cast<unsigned long long*>(base + 64);
+  {
+    size_t big = 0;  // tune on the largest box
+    for (size_t b = 1; b < descs.size(); ++b)
+      if (descs[b].ncells > descs[big].ncells) big = b;
+    int rc = autotune(h, P, descs[big], st);
+    if (rc) return rc;
+  }
   {
     const prk::BoxDesc* dboxes = nullptr;
     const long* dstart = nullptr;

@@ -289,3 +289,16 @@ def test_react_batch_graph_replay(gpu):
             for k in rf:
                 _assert_bits(f"replay {rep} {k}", t[k].cpu().numpy(), rf[k].cpu().numpy())
     r.close()
+
+
+def test_autotune_picks_a_variant_with_identical_results(gpu):
+    """hipbdf.autotune = 1: the first react call times the lanes-per-cell variants and block sizes
+    on a sample of the box and keeps the fastest; every variant computes the same bits, so the
+    tuned reactor's results equal a fixed 32-lane reactor's."""
+    box = W.ch4_air_flame_box(nx=64, ny=8, nz=8, seed=31)
+    fixed = _gpu(box, dict(KEYS, **{"cvode.max_order": 4, "hipbdf.lanes_per_cell": 32}), gpu)
+    tuned = _gpu(box, dict(KEYS, **{"cvode.max_order": 4, "hipbdf.autotune": 1}), gpu)
+    assert tuned["stats"]["lanes_per_cell"] in (16, 32, 64)
+    assert tuned["nfail"] == 0 and fixed["nfail"] == 0
+    for k in ("rY", "T", "rH", "FC"):
+        _assert_bits(k, tuned[k], fixed[k])
