@@ -812,15 +812,22 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
       const int owner = upper_half() ? o1 - 32 : o0;
       if (lane == owner) pst[0] = k;
       if (lane == k) permv = owner;
-      const double inv = 1.0 / hsel(rlane_d(lu[0][k], o0), rlane_d(lu[0][k], o1));
+      // the pivot row (final row k of U) goes through the group's LU area in LDS, which holds
+      // nothing during the factorization (the factors are stored after it)
+      if (lane == owner)
+#pragma unroll
+        for (int j = k; j < n; ++j) S.LU[k * n + j] = lu[0][j];
+      group_sync();
+      const double inv = 1.0 / S.LU[k * n + k];
       if (lane == owner) rinv[0] = inv;
       if (pst[0] > k) lu[0][k] *= inv;
 #pragma unroll
       for (int j = k + 1; j < n; ++j) {
-        const double pj = hsel(rlane_d(lu[0][j], o0), rlane_d(lu[0][j], o1));
+        const double pj = S.LU[k * n + j];
         if (pst[0] > k) lu[0][j] = fma(-lu[0][k], pj, lu[0][j]);
       }
     }
+    group_sync();
     return singular;
   }
 #pragma unroll
