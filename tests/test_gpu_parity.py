@@ -302,3 +302,67 @@ def test_autotune_picks_a_variant_with_identical_results(gpu):
     assert tuned["nfail"] == 0 and fixed["nfail"] == 0
     for k in ("rY", "T", "rH", "FC"):
         _assert_bits(k, tuned[k], fixed[k])
+
+
+def test_array4_views_inside_ghosted_fabs(gpu):
+    """Drop-in layout: the box [lo, hi] (lo != 0) is a view into FABs with two ghost cells per
+    side (AMReX Array4: p at the FAB's begin corner, FAB strides, component stride = FAB cells).
+    react and reactionRateRhoY on the views give the bits of contiguous runs on the same cells,
+    and the ghost cells (NaN) are neither read into results nor written."""
+    import ctypes as C
+    import torch
+    from pelelmex_amd import reactor as RM
+    from pelelmex_amd.reactor import Reactor
+    nx, ny, nz, ng = 8, 4, 4, 2
+    lo = (10, 20, 30)
+    box = W.ch4_air_flame_box(nx=nx, ny=ny, nz=nz, seed=41)
+    box["mask"][::6] = -1
+    N = box["rY"].shape[0]
+    FX, FY, FZ = nx + 2 * ng, ny + 2 * ng, nz + 2 * ng
+
+    def fab(a, ncomp, dtype=torch.float64, fill=float("nan")):
+        f = torch.full((ncomp, FZ, FY, FX), fill, dtype=dtype)
+        f[:, ng:ng + nz, ng:ng + ny, ng:ng + nx] = torch.as_tensor(np.asarray(a).reshape(ncomp, nz, ny, nx))
+        return f.to(gpu).contiguous()
+
+    f = {k: fab(box[k], N) for k in ("rY", "rYsrc")}
+    for k in ("T", "rH", "rHsrc"):
+        f[k] = fab(box[k], 1)
+    f["FC"] = fab(np.zeros(box["T"].size), 1)
+    fmask = fab(box["mask"], 1, dtype=torch.int32, fill=-7)
+    begin = (C.c_int * 3)(lo[0] - ng, lo[1] - ng, lo[2] - ng)
+
+    def a4(t, ncomp):
+        return RM.Array4(t.data_ptr(), FX, FX * FY, FX * FY * FZ, begin, ncomp)
+
+    r = Reactor.create("ReactorCvode", "drm19", dict(KEYS, **{"cvode.max_order": 4}))
+    r.init(2, 1)
+    L = RM.lib()
+    clo = (C.c_int * 3)(*lo)
+    chi = (C.c_int * 3)(lo[0] + nx - 1, lo[1] + ny - 1, lo[2] + nz - 1)
+    arrs = [a4(f["rY"], N), a4(f["rYsrc"], N), a4(f["T"], 1), a4(f["rH"], 1), a4(f["rHsrc"], 1), a4(f["FC"], 1)]
+    m = RM.IArray4(fmask.data_ptr(), FX, FX * FY, begin)
+    dt, tm = C.c_double(box["dt"]), C.c_double(0.0)
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rc = L.pr_react_box(r._h, clo, chi, *[C.byref(x) for x in arrs], C.byref(m), C.byref(dt), C.byref(tm), st, None)
+    assert rc == 0, L.pr_last_error()
+    ref = _gpu(box, dict(KEYS, **{"cvode.max_order": 4}), gpu)
+    inner = (slice(None), slice(ng, ng + nz), slice(ng, ng + ny), slice(ng, ng + nx))
+    for k, ncomp in (("rY", N), ("T", 1), ("rH", 1), ("FC", 1)):
+        got = f[k].cpu()
+        _assert_bits(k, got[inner].reshape(ncomp, -1).numpy().squeeze(), ref[k])
+        ghost = got.clone()
+        ghost[inner] = 0.0
+        assert torch.isnan(ghost).sum() == ghost.numel() - got[inner].numel(), f"{k}: a ghost cell was written"
+    # reactionRateRhoY on the same views (MKS inputs), against a contiguous call
+    fm = {k: fab(box[k] * s_, c_) for k, s_, c_ in (("rY", 1e3, N), ("rH", 0.1, 1), ("T", 1.0, 1))}
+    out = fab(np.zeros((N, box["T"].size)), N)
+    rc = L.pr_reaction_rate_box(r._h, clo, chi, C.byref(a4(fm["rY"], N)), C.byref(a4(fm["rH"], 1)),
+                                C.byref(a4(fm["T"], 1)), C.byref(a4(out, N)), C.byref(m), st)
+    assert rc == 0, L.pr_last_error()
+    dev = lambda a, *shape: torch.as_tensor(np.ascontiguousarray(a)).reshape(*shape).to(gpu).contiguous()
+    want = r.reaction_rate_rhoY(dev(box["rY"] * 1e3, N, nz, ny, nx), dev(box["rH"] * 0.1, nz, ny, nx),
+                                dev(box["T"], nz, ny, nx), mask=dev(box["mask"], nz, ny, nx))
+    torch.cuda.synchronize()
+    _assert_bits("rYdot", out.cpu()[inner].numpy(), want.cpu().numpy())
+    r.close()
