@@ -893,15 +893,24 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
 
 // The LU rows persist in the group's LDS area between factorizations; registers hold them only
 // while a factorization or a solve runs.
+// At 32 lanes (one row per lane) the rows are stored in pivot order (slot k = row pivoted at step
+// k), so that the solve finds the pivot row of step k in lane k.
 template <int N, int R, int G>
-__device__ __forceinline__ void store_lu(const Smem<N, R>& S, int lane, const double (&lu)[Cell<N, R, G>::RPL][N + 1]) {
+constexpr bool lu_pivot_order() {
+  return G == 32 && Cell<N, R, G>::RPL == 1;
+}
+
+template <int N, int R, int G>
+__device__ __forceinline__ void store_lu(const Smem<N, R>& S, int lane, const double (&lu)[Cell<N, R, G>::RPL][N + 1],
+                                         const int (&pst)[Cell<N, R, G>::RPL]) {
   constexpr int n = N + 1;
 #pragma unroll
   for (int s = 0; s < Cell<N, R, G>::RPL; ++s) {
     const int i = lane + G * s;
-    if (i < n)
+    const int slot = lu_pivot_order<N, R, G>() ? pst[s] : i;
+    if (i < n && slot >= 0 && slot < n)
 #pragma unroll
-      for (int j = 0; j < n; ++j) S.LU[i * n + j] = lu[s][j];
+      for (int j = 0; j < n; ++j) S.LU[slot * n + j] = lu[s][j];
   }
 }
 
@@ -940,21 +949,25 @@ __device__ __forceinline__ void lu_solve(const Smem<N, R>& S, int lane, const do
     }
     return;
   }
-  if constexpr (G == 32 && RPL == 1) {
+  if constexpr (lu_pivot_order<N, R, G>()) {
+    // lane k holds the row pivoted at step k (pst = k); its right-hand side entry is b[p_k]
+    double v = __shfl(b[0], permv, G);
+    const double rdiag = (lane < n) ? 1.0 / S.LU[lane * n + lane] : 0.0;  // the stored 1 / pivot
 #pragma unroll
     for (int k = 0; k < n; ++k) {
-      const int p0 = __builtin_amdgcn_readlane(permv, k), p1 = __builtin_amdgcn_readlane(permv, 32 + k) + 32;
-      const double yk = hsel(rlane_d(b[0], p0), rlane_d(b[0], p1));
-      if (pst[0] > k) b[0] = fma(-lu[0][k], yk, b[0]);
+      const double yk = hsel(rlane_d(v, k), rlane_d(v, k + 32));
+      if (lane > k && lane < n) v = fma(-lu[0][k], yk, v);
     }
     x[0] = 0.0;
 #pragma unroll
     for (int k = n - 1; k >= 0; --k) {
-      const int p0 = __builtin_amdgcn_readlane(permv, k), p1 = __builtin_amdgcn_readlane(permv, 32 + k) + 32;
-      const double xk = hsel(rlane_d(b[0], p0) * rlane_d(rinv[0], p0), rlane_d(b[0], p1) * rlane_d(rinv[0], p1));
-      if (pst[0] >= 0 && pst[0] < k) b[0] = fma(-lu[0][k], xk, b[0]);
+      const double w = v * rdiag;
+      const double xk = hsel(rlane_d(w, k), rlane_d(w, k + 32));
+      if (lane < k) v = fma(-lu[0][k], xk, v);
       if (lane == k) x[0] = xk;
     }
+    (void)pst;
+    (void)rinv;
     return;
   }
 #pragma unroll

@@ -572,7 +572,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2
         {
           double lu[RPL][n];
           singular = lu_factor<N, R, G>(S, lane, c.gamma, lu, pst, rinv, permv);
-          store_lu<N, R, G>(S, lane, lu);
+          store_lu<N, R, G>(S, lane, lu, pst);
         }
         clobber();
         c.nset += 1;
