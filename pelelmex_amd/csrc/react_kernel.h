@@ -77,8 +77,10 @@ __device__ __forceinline__ double* setup_views(Smem<N, R>& S, unsigned char* sme
   return scr;
 }
 
-template <int N, int R, int G>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) react_kernel(KParams P) {
+// W is the number of waves per SIMD the instance is compiled for: its VGPR budget is 512 / W
+// per lane and its blocks hold up to 256 W threads (W waves per SIMD from one block per CU).
+template <int N, int R, int G, int W>
+__global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W, W))) react_kernel(KParams P) {
   constexpr int n = N + 1;
   constexpr int RPL = Cell<N, R, G>::RPL;
   constexpr int Q = QMAXC;

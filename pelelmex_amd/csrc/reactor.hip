@@ -60,9 +60,9 @@ using TrustLaunch = hipError_t (*)(const prk::TrustParams&, int blocks, int bloc
 using FeatLaunch = hipError_t (*)(const prk::FeatParams&, hipStream_t);
 using PrepLaunch = hipError_t (*)(const prk::PrepParams&, int blocks, int block, size_t shmem, hipStream_t);
 
-template <int N, int R, int G>
+template <int N, int R, int G, int W>
 hipError_t launch_t(const prk::KParams& P, int blocks, int block, size_t shmem, hipStream_t st) {
-  hipLaunchKernelGGL((prk::react_kernel<N, R, G>), dim3(blocks), dim3(block), shmem, st, P);
+  hipLaunchKernelGGL((prk::react_kernel<N, R, G, W>), dim3(blocks), dim3(block), shmem, st, P);
   return hipGetLastError();
 }
 
@@ -104,10 +104,10 @@ hipError_t feat_t(const prk::FeatParams& P, hipStream_t st) {
   return hipGetLastError();
 }
 
-template <int N, int R, int G>
+template <int N, int R, int G, int W>
 hipError_t occ_t(int* nb, int block, size_t shmem) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      nb, reinterpret_cast<const void*>(&prk::react_kernel<N, R, G>), block, shmem);
+      nb, reinterpret_cast<const void*>(&prk::react_kernel<N, R, G, W>), block, shmem);
 }
 
 template <int N, int R, int G>
@@ -118,6 +118,7 @@ constexpr int scratch_bytes() {
 struct Variant {
   const char* mech;
   int G;
+  int waves;    // waves per SIMD react_kernel is compiled for; blocks of up to 256 * waves threads
   Launch launch;
   EvalLaunch eval;
   Occ occ;
@@ -129,16 +130,20 @@ struct Variant {
   PrepLaunch prep;
 };
 
-#define PR_VARIANT(NAME, NS, NR, G) \
-  {NAME, G, &launch_t<NS, NR, G>, &eval_t<NS, NR, G>, &occ_t<NS, NR, G>, scratch_bytes<NS, NR, G>(), \
+#define PR_VARIANT(NAME, NS, NR, G, W) \
+  {NAME, G, W, &launch_t<NS, NR, G, W>, &eval_t<NS, NR, G>, &occ_t<NS, NR, G, W>, scratch_bytes<NS, NR, G>(), \
    prk::Cell<NS, NR, G>::gslot_doubles * 8, &rate_t<NS, NR, G>, &trust_t<NS, NR, G>, &feat_t<NS>, &prep_t<NS, NR, G>}
 
+// drm19 at 32 lanes runs 3 waves per SIMD (24 cells in a 768-thread block, 154 KB of LDS per
+// CU): some registers spill, yet it is 18% faster than 2 waves with none (the integrator is
+// latency bound). At 64 lanes the group's LDS does not fit 3 waves; at 16 lanes 4 cells share
+// a wave and registers already spill at 2.
 const Variant kVariants[] = {
-    PR_VARIANT("LiDryer", PR_LIDRYER_NSP, PR_LIDRYER_NR, 16),
-    PR_VARIANT("LiDryer", PR_LIDRYER_NSP, PR_LIDRYER_NR, 8),
-    PR_VARIANT("drm19", PR_DRM19_NSP, PR_DRM19_NR, 32),
-    PR_VARIANT("drm19", PR_DRM19_NSP, PR_DRM19_NR, 16),
-    PR_VARIANT("drm19", PR_DRM19_NSP, PR_DRM19_NR, 64),
+    PR_VARIANT("LiDryer", PR_LIDRYER_NSP, PR_LIDRYER_NR, 16, 2),
+    PR_VARIANT("LiDryer", PR_LIDRYER_NSP, PR_LIDRYER_NR, 8, 2),
+    PR_VARIANT("drm19", PR_DRM19_NSP, PR_DRM19_NR, 32, 3),
+    PR_VARIANT("drm19", PR_DRM19_NSP, PR_DRM19_NR, 16, 2),
+    PR_VARIANT("drm19", PR_DRM19_NSP, PR_DRM19_NR, 64, 2),
 };
 
 const Variant* find_variant(const char* mech, int G) {
@@ -297,7 +302,8 @@ int parse_keys(const char* text, Keys& K) {
       if (!num(d) || (d != 0 && d != 1)) return fail(PR_ERR_KEY, "hipbdf.autotune must be 0 or 1");
       K.autotune = (int)d;
     } else if (k == "hipbdf.block") {
-      if (!num(d) || ((int)d % 64) != 0 || d < 64 || d > 256) return fail(PR_ERR_KEY, "hipbdf.block must be 64, 128, 192 or 256");
+      if (!num(d) || ((int)d % 64) != 0 || d < 64 || d > 768)
+        return fail(PR_ERR_KEY, "hipbdf.block must be a multiple of 64 from 64 to 768");
       K.block = (int)d;
     } else if (k == "surrogate.enable") {
       if (!num(d)) return fail(PR_ERR_KEY, "bad surrogate.enable");
@@ -436,11 +442,29 @@ int pr_num_species(const pr_handle* h) { return h ? h->mech->nsp : PR_ERR_ARG; }
 
 // Largest block (64..256 threads) whose LDS (tables + per-group scratch) stays within 80 KiB,
 // so that two blocks share a CU's 160 KiB.
+// LDS per CU on gfx950
+constexpr size_t kLdsPerCU = 163840;
+
+// Block size of the one-cell-per-group helper kernels (launch bound prk::BLOCK): the largest
+// that lets two blocks share a CU's LDS.
 static int pick_block(const pr_handle* h) {
-  if (h->block) return h->block;
+  if (h->block && h->block <= prk::BLOCK) return h->block;
   const int G = h->var->G;
-  for (int b = 256; b >= 64; b -= 64)
-    if ((size_t)h->blob_bytes + (size_t)(b / G) * h->var->scratch <= 81920) return b;
+  for (int b = prk::BLOCK; b >= 64; b -= 64)
+    if ((size_t)h->blob_bytes + (size_t)(b / G) * h->var->scratch <= kLdsPerCU / 2) return b;
+  return 64;
+}
+
+// Block size of the BDF kernel: hipbdf.block when set, else the largest block (up to
+// 256 * waves threads) whose LDS (mechanism tables + per-group areas) lets the variant's waves
+// per SIMD be resident, i.e. ceil(256 * waves / b) blocks per CU.
+static int bdf_block(const pr_handle* h) {
+  if (h->block) return h->block;
+  const int G = h->var->G, W = h->var->waves;
+  for (int b = 256 * W; b >= 64; b -= 64) {
+    const size_t per_cu = (size_t)((256 * W + b - 1) / b);
+    if ((size_t)h->blob_bytes + (size_t)(b / G) * h->var->scratch <= kLdsPerCU / per_cu) return b;
+  }
   return 64;
 }
 
@@ -473,7 +497,10 @@ static prk::Arr to_arr(const pr_array4* a) {
 // and P.nfail first (all on the stream).
 static int bdf_launch(pr_handle* h, prk::KParams& P, long work_max, hipStream_t st) {
   const int G = h->var->G;
-  const int block = pick_block(h);
+  const int block = bdf_block(h);
+  if (block > 256 * h->var->waves)
+    return fail(PR_ERR_KEY, "hipbdf.block " + std::to_string(block) + " exceeds " + std::to_string(256 * h->var->waves) +
+                                " threads for " + std::to_string(G) + " lanes per cell");
   const size_t shmem = (size_t)h->blob_bytes + (size_t)(block / G) * h->var->scratch;
   int dev_cus = 0, nb = 0;
   HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, h->device));
@@ -606,8 +633,8 @@ static int mlp_forward(pr_handle* h, long ncells, hipStream_t st) {
   return 0;
 }
 
-// hipbdf.autotune: on the first react call, every lanes-per-cell variant of the mechanism and
-// block sizes of 128 and 256 threads integrate the same evenly spaced sample (up to 65536
+// hipbdf.autotune: on the first react call, every lanes-per-cell variant of the mechanism, with
+// its own block size and with 128 and 256 threads, integrates the same evenly spaced sample (up to 65536
 // cells) of the box (a copy:
 // the box itself is untouched); the fastest per cell is kept for the handle. All variants give
 // identical results, so the choice only affects speed.
@@ -657,8 +684,10 @@ static int autotune(pr_handle* h, const prk::KParams& Pbase, const prk::BoxDesc&
   for (const Variant& v : kVariants) {
     if (e != hipSuccess) break;
     if (std::strcmp(v.mech, h->mech->name) != 0) continue;
-    for (int blk : {128, 256}) {
+    for (int blk : {0, 256, 128}) {  // 0: the variant's own choice (bdf_block)
       h->var = &v;
+      h->block = 0;
+      if (blk != 0 && blk == bdf_block(h)) continue;
       h->block = blk;
       prk::KParams P = Pbase;
       const prk::BoxDesc* db = nullptr;

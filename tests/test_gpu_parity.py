@@ -304,6 +304,23 @@ def test_autotune_picks_a_variant_with_identical_results(gpu):
         _assert_bits(k, tuned[k], fixed[k])
 
 
+def test_block_size_changes_nothing_but_speed(gpu):
+    """The 32-lane drm19 kernel runs 3 waves per SIMD in 768-thread blocks by default;
+    hipbdf.block = 256 and 128 give the same bits. A block above a variant's limit (768 threads
+    at 64 lanes, compiled for 2 waves per SIMD) is refused before anything is launched."""
+    from pelelmex_amd.reactor import ReactorError
+    box = W.ch4_air_flame_box(nx=32, ny=8, nz=8, seed=37)
+    base = dict(KEYS, **{"cvode.max_order": 4, "hipbdf.lanes_per_cell": 32})
+    auto = _gpu(box, base, gpu)
+    assert auto["nfail"] == 0
+    for blk in (256, 128):
+        other = _gpu(box, dict(base, **{"hipbdf.block": blk}), gpu)
+        for k in ("rY", "T", "rH", "FC"):
+            _assert_bits(k, other[k], auto[k])
+    with pytest.raises(ReactorError, match="exceeds"):
+        _gpu(box, dict(base, **{"hipbdf.lanes_per_cell": 64, "hipbdf.block": 768}), gpu)
+
+
 def test_array4_views_inside_ghosted_fabs(gpu):
     """Drop-in layout: the box [lo, hi] (lo != 0) is a view into FABs with two ghost cells per
     side (AMReX Array4: p at the FAB's begin corner, FAB strides, component stride = FAB cells).
