@@ -219,7 +219,8 @@ def main():
                      "frac": achieved / roofline.MI355X_FP64_TFLOPS, "traffic": traffic,
                      "traffic_source": (f"profiles/{traffic_src} (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a "
                                         "1 GiB device copy)") if traffic else "no profile of these kernel sources",
-                     "kernel": "react_kernel<21,84,%d> (plain BDF over the whole box)" % st["lanes_per_cell"],
+                     "kernel": "react_kernel<21,84,%d,%d> (plain BDF over the whole box, %d-thread blocks)" % (
+                         st["lanes_per_cell"], st["waves_per_simd"], st["block_threads"]),
                      "kernel_ms": kernel_ms, "flops_per_launch": F,
                      "hbm_algorithmic_bytes": counts["bytes_cell"] * ncells,
                      "hbm_algorithmic_GBs": counts["bytes_cell"] * ncells / (kernel_ms / 1e3) / 1e9},

@@ -68,6 +68,7 @@ typedef struct {
   double surrogate_ms;  /* features + network + trust check */
   double audit_ms;      /* BDF re-solve of the audit sample */
   double audit_max_dT_rel, audit_max_dY;  /* worst |dT| / T and |dY_k| of audited cells vs BDF */
+  int block_threads, waves_per_simd;      /* BDF kernel launch shape */
 } pr_stats;
 
 /* Error codes (negative); pr_react_box returns >= 0 (number of failed cells) on success. */

@@ -146,6 +146,13 @@ __device__ __forceinline__ double pr_log(double x) {
 __device__ __forceinline__ double pr_log10(double x) { return pr_log(x) * PRK_INV_LN10; }
 __device__ __forceinline__ double pr_pow(double x, double y) { return pr_exp(y * pr_log(x)); }
 
+// v through an empty volatile asm: expressions of it are computed where they are used, not
+// hoisted out of the integrator loop into registers held for the whole kernel.
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // ---------------------------------------------------------------- group primitives
 __device__ __forceinline__ void group_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -758,8 +765,9 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
     // The pivot owner is wave-uniform: the pivot row is read with readlane, no LDS round trips.
     pst[0] = (lane < n) ? UNPIV : -1;
     rinv[0] = 0.0;
+    const int ol = opaque(lane);
 #pragma unroll
-    for (int j = 0; j < n; ++j) lu[0][j] = (lane < n) ? ((lane == j ? 1.0 : 0.0) - gamma * S.Js[lane * n + j]) : 0.0;
+    for (int j = 0; j < n; ++j) lu[0][j] = (lane < n) ? ((ol == j ? 1.0 : 0.0) - gamma * S.Js[lane * n + j]) : 0.0;
     bool singular = false;
 #pragma unroll
     for (int k = 0; k < n; ++k) {
@@ -791,8 +799,9 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
     // as above, one pivot owner per half; the pivot row is read with readlane from both halves
     pst[0] = (lane < n) ? UNPIV : -1;
     rinv[0] = 0.0;
+    const int ol = opaque(lane);
 #pragma unroll
-    for (int j = 0; j < n; ++j) lu[0][j] = (lane < n) ? ((lane == j ? 1.0 : 0.0) - gamma * S.Js[lane * n + j]) : 0.0;
+    for (int j = 0; j < n; ++j) lu[0][j] = (lane < n) ? ((ol == j ? 1.0 : 0.0) - gamma * S.Js[lane * n + j]) : 0.0;
     bool singular = false;
 #pragma unroll
     for (int k = 0; k < n; ++k) {
@@ -835,8 +844,9 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
     const int i = lane + G * s;
     pst[s] = (i < n) ? UNPIV : -1;
     rinv[s] = 0.0;
+    const int oi = opaque(i);
 #pragma unroll
-    for (int j = 0; j < n; ++j) lu[s][j] = (i < n) ? ((i == j ? 1.0 : 0.0) - gamma * S.Js[i * n + j]) : 0.0;
+    for (int j = 0; j < n; ++j) lu[s][j] = (i < n) ? ((oi == j ? 1.0 : 0.0) - gamma * S.Js[i * n + j]) : 0.0;
   }
   bool singular = false;
 #pragma unroll

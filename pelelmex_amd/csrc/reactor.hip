@@ -520,6 +520,8 @@ static int bdf_launch(pr_handle* h, prk::KParams& P, long work_max, hipStream_t 
   HIPCHK(h->var->launch(P, blocks, block, shmem, st));
   h->last.lanes_per_cell = G;
   h->last.blocks = blocks;
+  h->last.block_threads = block;
+  h->last.waves_per_simd = h->var->waves;
   return 0;
 }
 
@@ -1103,9 +1105,12 @@ int pr_collect(pr_handle* h, void* stream) {
   HIPCHK(hipMemcpyAsync(&nfail, base + 4, sizeof(nfail), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   const int G = h->last.lanes_per_cell, blocks = h->last.blocks;
+  const int bt = h->last.block_threads, wv = h->last.waves_per_simd;
   h->last = pr_stats{};
   h->last.lanes_per_cell = G;
   h->last.blocks = blocks;
+  h->last.block_threads = bt;
+  h->last.waves_per_simd = wv;
   h->last.cells = (int64_t)hs[0];
   h->last.nst = (int64_t)hs[1];
   h->last.nfe = (int64_t)hs[2];

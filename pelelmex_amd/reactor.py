@@ -51,7 +51,7 @@ class Stats(C.Structure):
                 ("blocks", C.c_int), ("sur_accepted", C.c_int64), ("sur_rejected", C.c_int64),
                 ("sur_audited", C.c_int64), ("audit_failed", C.c_int64), ("total_ms", C.c_double),
                 ("surrogate_ms", C.c_double), ("audit_ms", C.c_double), ("audit_max_dT_rel", C.c_double),
-                ("audit_max_dY", C.c_double)]
+                ("audit_max_dY", C.c_double), ("block_threads", C.c_int), ("waves_per_simd", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
