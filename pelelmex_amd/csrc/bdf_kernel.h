@@ -272,8 +272,11 @@ __device__ __forceinline__ double gpsum(int lane, const double (&a)[RPL]) {
   double t[NB];
 #pragma unroll
   for (int s = 0; s < NB; ++s) {
-    double v = 0.0;
-    if (s < RPL) v = (lane + G * s < CNT) ? a[s < RPL ? s : 0] : 0.0;
+    if (s >= RPL || G * s >= CNT) {  // an all-zero block sums to +0.0
+      t[s] = 0.0;
+      continue;
+    }
+    double v = (lane + G * s < CNT) ? a[s] : 0.0;
     if constexpr (G == 32) {
       v = half_reduce<false>(v);  // the same pairwise tree over the 32 lanes of the half
     } else {
@@ -338,7 +341,16 @@ __device__ __forceinline__ void rx_eval(const RxRec& rx, const double* sC, const
   if (type != PR_RX_ELEM) {
     M = rx.effd * Ctot;
     const int eb = rx.eff & 0xFFFF, ee = (rx.eff >> 16) & 0xFFFF;
-    for (int e = eb; e < ee; ++e) M = fma(effval[e], sC[effsp[e]], M);
+    // efficiency list in its order; the table reads of four entries are issued together
+    for (int e = eb; e < ee; e += 4) {
+      const int e1 = (e + 1 < ee) ? e + 1 : e, e2 = (e + 2 < ee) ? e + 2 : e, e3 = (e + 3 < ee) ? e + 3 : e;
+      const double c0 = sC[effsp[e]], c1 = sC[effsp[e1]], c2 = sC[effsp[e2]], c3 = sC[effsp[e3]];
+      const double v0 = effval[e], v1 = effval[e1], v2 = effval[e2], v3 = effval[e3];
+      M = fma(v0, c0, M);
+      if (e + 1 < ee) M = fma(v1, c1, M);
+      if (e + 2 < ee) M = fma(v2, c2, M);
+      if (e + 3 < ee) M = fma(v3, c3, M);
+    }
   }
   const int r0 = rx.reac & 0xFF, r1 = (rx.reac >> 8) & 0xFF, r2 = (rx.reac >> 16) & 0xFF;
   const int p0 = rx.prod & 0xFF, p1 = (rx.prod >> 8) & 0xFF, p2 = (rx.prod >> 16) & 0xFF;
