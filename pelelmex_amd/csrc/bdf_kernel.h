@@ -779,7 +779,8 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
     rinv[0] = 0.0;
     const int ol = opaque(lane);
 #pragma unroll
-    for (int j = 0; j < n; ++j) lu[0][j] = (lane < n) ? ((ol == j ? 1.0 : 0.0) - gamma * S.Js[lane * n + j]) : 0.0;
+    for (int j = 0; j < n; ++j)  // lanes >= n hold a copy of row n - 1, never read (pst = -1)
+      lu[0][j] = (ol == j ? 1.0 : 0.0) - gamma * S.Js[(lane < n ? lane : n - 1) * n + j];
     bool singular = false;
 #pragma unroll
     for (int k = 0; k < n; ++k) {
@@ -813,7 +814,8 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
     rinv[0] = 0.0;
     const int ol = opaque(lane);
 #pragma unroll
-    for (int j = 0; j < n; ++j) lu[0][j] = (lane < n) ? ((ol == j ? 1.0 : 0.0) - gamma * S.Js[lane * n + j]) : 0.0;
+    for (int j = 0; j < n; ++j)  // lanes >= n hold a copy of row n - 1, never read (pst = -1)
+      lu[0][j] = (ol == j ? 1.0 : 0.0) - gamma * S.Js[(lane < n ? lane : n - 1) * n + j];
     bool singular = false;
 #pragma unroll
     for (int k = 0; k < n; ++k) {
@@ -841,11 +843,11 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
       group_sync();
       const double inv = 1.0 / S.LU[k * n + k];
       if (lane == owner) rinv[0] = inv;
-      if (pst[0] > k) lu[0][k] *= inv;
+      if (pst[0] > k) {  // rows still to be pivoted: one masked region, no per-element selects
+        lu[0][k] *= inv;
+        const double m = lu[0][k];
 #pragma unroll
-      for (int j = k + 1; j < n; ++j) {
-        const double pj = S.LU[k * n + j];
-        if (pst[0] > k) lu[0][j] = fma(-lu[0][k], pj, lu[0][j]);
+        for (int j = k + 1; j < n; ++j) lu[0][j] = fma(-m, S.LU[k * n + j], lu[0][j]);
       }
     }
     group_sync();
@@ -858,7 +860,8 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
     rinv[s] = 0.0;
     const int oi = opaque(i);
 #pragma unroll
-    for (int j = 0; j < n; ++j) lu[s][j] = (i < n) ? ((oi == j ? 1.0 : 0.0) - gamma * S.Js[i * n + j]) : 0.0;
+    for (int j = 0; j < n; ++j)  // rows >= n hold a copy of row n - 1, never read (pst = -1)
+      lu[s][j] = (oi == j ? 1.0 : 0.0) - gamma * S.Js[(i < n ? i : n - 1) * n + j];
   }
   bool singular = false;
 #pragma unroll
@@ -943,7 +946,7 @@ __device__ __forceinline__ void load_lu(const Smem<N, R>& S, int lane, double (&
   for (int s = 0; s < Cell<N, R, G>::RPL; ++s) {
     const int i = lane + G * s;
 #pragma unroll
-    for (int j = 0; j < n; ++j) lu[s][j] = (i < n) ? S.LU[i * n + j] : 0.0;
+    for (int j = 0; j < n; ++j) lu[s][j] = S.LU[(i < n ? i : n - 1) * n + j];  // rows >= n: never read
   }
 }
 
