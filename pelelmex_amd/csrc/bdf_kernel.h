@@ -90,6 +90,14 @@ enum { ST_OK = 0, ST_TOO_MUCH_WORK = -1, ST_ERR_FAILURE = -3, ST_CONV_FAILURE = 
 #define PRK_LN10 2.30258509299404590109e+00
 #define PRK_INV_LN10 4.34294481903251816668e-01
 
+// A constant materialized in scalar registers where it is used (an empty asm that may change it):
+// polynomial steps then issue as one v_fma_f64 with a scalar addend, instead of a vector copy of
+// a constant hoisted into vector registers followed by v_fmac_f64.
+__device__ __forceinline__ double kc(double c) {
+  asm volatile("" : "+s"(c));
+  return c;
+}
+
 __device__ __forceinline__ double pr_exp(double x) {
   if (x != x) return x;
   if (x > 709.782712893384) return __builtin_huge_val();
@@ -98,16 +106,16 @@ __device__ __forceinline__ double pr_exp(double x) {
   double r = fma(-k, PRK_LN2_HI, x);
   r = fma(-k, PRK_LN2_LO, r);
   double p = 1.0 / 6227020800.0;
-  p = fma(p, r, 1.0 / 479001600.0);
-  p = fma(p, r, 1.0 / 39916800.0);
-  p = fma(p, r, 1.0 / 3628800.0);
-  p = fma(p, r, 1.0 / 362880.0);
-  p = fma(p, r, 1.0 / 40320.0);
-  p = fma(p, r, 1.0 / 5040.0);
-  p = fma(p, r, 1.0 / 720.0);
-  p = fma(p, r, 1.0 / 120.0);
-  p = fma(p, r, 1.0 / 24.0);
-  p = fma(p, r, 1.0 / 6.0);
+  p = fma(p, r, kc(1.0 / 479001600.0));
+  p = fma(p, r, kc(1.0 / 39916800.0));
+  p = fma(p, r, kc(1.0 / 3628800.0));
+  p = fma(p, r, kc(1.0 / 362880.0));
+  p = fma(p, r, kc(1.0 / 40320.0));
+  p = fma(p, r, kc(1.0 / 5040.0));
+  p = fma(p, r, kc(1.0 / 720.0));
+  p = fma(p, r, kc(1.0 / 120.0));
+  p = fma(p, r, kc(1.0 / 24.0));
+  p = fma(p, r, kc(1.0 / 6.0));
   p = fma(p, r, 0.5);
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
@@ -127,15 +135,15 @@ __device__ __forceinline__ double pr_log(double x) {
   const double s = f / (2.0 + f);
   const double z = s * s;
   double p = 1.0 / 21.0;
-  p = fma(p, z, 1.0 / 19.0);
-  p = fma(p, z, 1.0 / 17.0);
-  p = fma(p, z, 1.0 / 15.0);
-  p = fma(p, z, 1.0 / 13.0);
-  p = fma(p, z, 1.0 / 11.0);
-  p = fma(p, z, 1.0 / 9.0);
-  p = fma(p, z, 1.0 / 7.0);
-  p = fma(p, z, 1.0 / 5.0);
-  p = fma(p, z, 1.0 / 3.0);
+  p = fma(p, z, kc(1.0 / 19.0));
+  p = fma(p, z, kc(1.0 / 17.0));
+  p = fma(p, z, kc(1.0 / 15.0));
+  p = fma(p, z, kc(1.0 / 13.0));
+  p = fma(p, z, kc(1.0 / 11.0));
+  p = fma(p, z, kc(1.0 / 9.0));
+  p = fma(p, z, kc(1.0 / 7.0));
+  p = fma(p, z, kc(1.0 / 5.0));
+  p = fma(p, z, kc(1.0 / 3.0));
   const double t = z * p;
   const double s2 = s + s;
   const double logm = fma(s2, t, s2);
