@@ -195,6 +195,13 @@ __device__ __forceinline__ double xor16_d(double v) {
   return __hiloint2double(up ? b[0] : b[1], up ? a[0] : a[1]);
 }
 
+// value of the lane whose byte address (4 x lane) is addr, through the LDS crossbar
+__device__ __forceinline__ double bperm_d(double v, int addr) {
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, __double2loint(v));
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
 // Two cells per wavefront (G == 32): lane l of the group is wave lane l (lanes 0..31) or
 // 32 + l (lanes 32..63). hsel picks this lane's half of a pair of wave-uniform values.
 __device__ __forceinline__ bool upper_half() { return (threadIdx.x & 32) != 0; }
@@ -983,22 +990,23 @@ __device__ __forceinline__ void lu_solve(const Smem<N, R>& S, int lane, const do
     return;
   }
   if constexpr (lu_pivot_order<N, R, G>()) {
-    // lane k holds the row pivoted at step k (pst = k); its right-hand side entry is b[p_k]
+    // lane k holds the row pivoted at step k (pst = k); its right-hand side entry is b[p_k].
+    // Step k's value comes from lane k of the half through the LDS crossbar (ds_bpermute, the
+    // lane in the immediate offset): no vector instructions for the broadcast.
+    const int base = (int)(threadIdx.x & 32) * 4;
     double v = __shfl(b[0], permv, G);
     const double rdiag = (lane < n) ? 1.0 / S.LU[lane * n + lane] : 0.0;  // the stored 1 / pivot
 #pragma unroll
     for (int k = 0; k < n; ++k) {
-      const double yk = hsel(rlane_d(v, k), rlane_d(v, k + 32));
+      const double yk = bperm_d(v, base + 4 * k);
       if (lane > k && lane < n) v = fma(-lu[0][k], yk, v);
     }
-    x[0] = 0.0;
 #pragma unroll
-    for (int k = n - 1; k >= 0; --k) {
-      const double w = v * rdiag;
-      const double xk = hsel(rlane_d(w, k), rlane_d(w, k + 32));
+    for (int k = n - 1; k >= 1; --k) {
+      const double xk = bperm_d(v * rdiag, base + 4 * k);
       if (lane < k) v = fma(-lu[0][k], xk, v);
-      if (lane == k) x[0] = xk;
     }
+    x[0] = v * rdiag;  // lane k's v is final from step k on: x_k = v_k / pivot_k
     (void)pst;
     (void)rinv;
     return;
