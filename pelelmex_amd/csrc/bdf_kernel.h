@@ -849,7 +849,7 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
       const int o0 = m0 ? __builtin_ctz(m0) : 0, o1 = (m1 ? __builtin_ctz(m1) : 0) + 32;  // first maximal rows
       const int owner = upper_half() ? o1 - 32 : o0;
       if (lane == owner) pst[0] = k;
-      if (lane == k) permv = owner;
+      if (ol == k) permv = owner;
       // the pivot row (final row k of U) goes through the group's LU area in LDS, which holds
       // nothing during the factorization (the factors are stored after it)
       if (lane == owner)
@@ -994,17 +994,18 @@ __device__ __forceinline__ void lu_solve(const Smem<N, R>& S, int lane, const do
     // Step k's value comes from lane k of the half through the LDS crossbar (ds_bpermute, the
     // lane in the immediate offset): no vector instructions for the broadcast.
     const int base = (int)(threadIdx.x & 32) * 4;
+    const int ol = opaque(lane);  // the step masks are formed here, not hoisted and spilled
     double v = __shfl(b[0], permv, G);
     const double rdiag = (lane < n) ? 1.0 / S.LU[lane * n + lane] : 0.0;  // the stored 1 / pivot
 #pragma unroll
     for (int k = 0; k < n; ++k) {
       const double yk = bperm_d(v, base + 4 * k);
-      if (lane > k && lane < n) v = fma(-lu[0][k], yk, v);
+      if (ol > k && ol < n) v = fma(-lu[0][k], yk, v);
     }
 #pragma unroll
     for (int k = n - 1; k >= 1; --k) {
       const double xk = bperm_d(v * rdiag, base + 4 * k);
-      if (lane < k) v = fma(-lu[0][k], xk, v);
+      if (ol < k) v = fma(-lu[0][k], xk, v);
     }
     x[0] = v * rdiag;  // lane k's v is final from step k on: x_k = v_k / pivot_k
     (void)pst;

@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
   __syncthreads();
   const BlobLayout BL = blob_layout(N, R, P.neff, P.ninc);
   Smem<N, R> S;
-  const int lane = threadIdx.x % G;
+  int lane = threadIdx.x % G;  // re-derived opaquely each iteration (see the loop head)
   const int grp = threadIdx.x / G;
   Ctl& c = *reinterpret_cast<Ctl*>(setup_views<N, R, G>(S, smem, BL, grp, P.gscr));
 
@@ -403,6 +403,9 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
 
   for (;;) {
     clobber();
+    // lane-derived masks and addresses are formed inside each iteration: hoisted out of the loop
+    // they would be held (and spilled) for the whole kernel
+    lane = opaque(lane);
     if (uni<G>(c.phase) == PH_FETCH) {
       unsigned int cl = 0;
       if (lane == 0) cl = atomicAdd(P.queue, 1u);
