@@ -28,6 +28,11 @@
  *   pr_reaction_rate_box  reactionRateRhoY over a box (Source/PeleLMeX_K.H:848-889), as
  *                    PeleLM::computeInstantaneousReactionRate runs it per box
  *                    (Source/PeleLMeX_Reactions.cpp:399-435)
+ *   pr_set_temperature_box  PeleLM::setTemperature per box (Source/PeleLMeX_Eos.cpp:218-240,
+ *                    getTfromHY Source/PeleLMeX_K.H:700-723)
+ *   pr_floor_species_box    PeleLM::floorSpecies per box (Source/PeleLMeX_Utils.cpp:649-690)
+ *   pr_heat_release_box     PeleLM::getHeatRelease per box (Source/PeleLMeX_Reactions.cpp:477-506,
+ *                    getHGivenT Source/PeleLMeX_K.H:598-615)
  *   pr_surrogate_load, pr_surrogate_features, pr_mlp_*   no reference counterpart: the learned
  *                    dt_react map that pr_react_box tries before the BDF (cells failing its FP64
  *                    trust check are integrated by the BDF), and the kernels that train it
@@ -131,6 +136,23 @@ int pr_jac_batch(pr_handle* h, int64_t ncells, const double* y, const double* rY
 int pr_reaction_rate_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY,
                          const pr_array4* rH, const pr_array4* T, const pr_array4* rYdot,
                          const pr_iarray4* mask, void* stream);
+/* Equation-of-state box kernels around the chemistry, MKS Array4 views as PeleLMeX's state
+   (rho kg m^-3, rY kg m^-3, rH J m^-3, T K), one enqueue on `stream` then a wait for it.
+   pr_set_temperature_box (setTemperature, Source/PeleLMeX_Eos.cpp:218-240 with getTfromHY,
+   Source/PeleLMeX_K.H:700-723): T = HY2T(rH / rho * 1e4, rY / rho) from the stored T (the
+   reactor's own temperature iteration); reads rho, rY, rH, T, writes T.
+   pr_floor_species_box (floorSpecies, Source/PeleLMeX_Utils.cpp:649-690): rY_k = max(0,
+   min(DBL_MAX, rY_k)), rho = sum_k rY_k, rH = TY2H(T, rY / rho) * 1e-4 * rho; reads rY, T,
+   writes rY, rho, rH.
+   pr_heat_release_box (getHeatRelease, Source/PeleLMeX_Reactions.cpp:477-506): HR = -sum_k
+   h_k(T) I_R,k with h_k in J kg^-1 (getHGivenT, Source/PeleLMeX_K.H:598-615) and I_R in
+   kg m^-3 s^-1; reads T, IR, writes HR (W m^-3). */
+int pr_set_temperature_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rho,
+                           const pr_array4* rY, const pr_array4* rH, const pr_array4* T, void* stream);
+int pr_floor_species_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY,
+                         const pr_array4* rho, const pr_array4* rH, const pr_array4* T, void* stream);
+int pr_heat_release_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* T,
+                        const pr_array4* IR, const pr_array4* HR, void* stream);
 /* Learned dt_react map. Network: nlayers dense layers, dims[0] = NUM_SPECIES + 2 inputs
    (log10(Y_k + 1e-20), T / 2000, log10(dt)), dims[nlayers] = NUM_SPECIES + 1 outputs (dY_k, dT),
    GELU between layers. params (host, fp32): for each layer W [dims[l] x dims[l+1]] row-major then

@@ -23,6 +23,7 @@
  * kernels evaluate the same trees, so GPU and oracle agree bit for bit; the oracle itself is
  * pinned by tests against finite differences, conservation laws and scipy integrators.
  */
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -384,6 +385,69 @@ void orc_rate_rhoY(int id, long ncells, const double* rY, const double* rH, cons
     gather_wdot(m, q, wdot);
     for (int k = 0; k < N; ++k) out[(long)k * ncells + c] = (wdot[k] * m->W[k]) * 1000.0;
     if (Tout) Tout[c] = T;
+  }
+}
+
+/* setTemperature (Source/PeleLMeX_Eos.cpp:218-240) through getTfromHY (Source/PeleLMeX_K.H:700-723),
+   MKS, component-major rY: rhoinv = 1 / rho (the DENSITY component), Y = rhoY * rhoinv,
+   h = rhoH * rhoinv * 1e4 (erg/g), T = HY2T(h, Y) started at the stored T, written back. */
+void orc_set_temperature(int id, long ncells, const double* rho, const double* rY, const double* rH, double* T) {
+  const int N = MECHS[id]->nsp;
+  for (long c = 0; c < ncells; ++c) {
+    double Y[ORC_NMAX];
+    const double rhoinv = 1.0 / rho[c];
+    for (int k = 0; k < N; ++k) Y[k] = rY[(long)k * ncells + c] * rhoinv;
+    double t = T[c];
+    hy2t_rhoY(id, (rH[c] * rhoinv) * 1.0e4, Y, 1.0, &t);
+    T[c] = t;
+  }
+}
+
+/* floorSpecies (Source/PeleLMeX_Utils.cpp:649-690): fabMinMax (Source/PeleLMeX_K.H:894-907)
+   rhoY_k = max(0, min(DBL_MAX, rhoY_k)) with std::min / std::max semantics ((b < a) ? b : a and
+   (a < b) ? b : a, so NaN becomes DBL_MAX and -0 becomes +0); rho = 0 + rhoY_0 + rhoY_1 + ... in
+   species order; Y = rhoY * (1 / rho); h = TY2H(T, Y) = sum over species in order of
+   ((Y_k h_k/RT) R/W_k) T; rhoH = h * 1e-4 * rho. Updates rY, writes rho and rH. */
+void orc_floor_species(int id, long ncells, double* rY, double* rho, double* rH, const double* T) {
+  const int N = MECHS[id]->nsp;
+  for (long c = 0; c < ncells; ++c) {
+    double r = 0.0;
+    for (int k = 0; k < N; ++k) {
+      double x = rY[(long)k * ncells + c];
+      x = (x < DBL_MAX) ? x : DBL_MAX;
+      x = (0.0 < x) ? x : 0.0;
+      rY[(long)k * ncells + c] = x;
+      r = r + x;
+    }
+    const double rhoinv = 1.0 / r;
+    const double t = T[c], invT = 1.0 / t;
+    double h = 0.0;
+    for (int k = 0; k < N; ++k) {
+      const sp_t* p = &SP[id][k];
+      double cpR, hRT;
+      nasa_hcp(p, t, invT, &cpR, &hRT);
+      h = h + (((rY[(long)k * ncells + c] * rhoinv) * hRT) * p->RW) * t;
+    }
+    rho[c] = r;
+    rH[c] = (h * 1.0e-4) * r;
+  }
+}
+
+/* getHeatRelease (Source/PeleLMeX_Reactions.cpp:477-506) with getHGivenT
+   (Source/PeleLMeX_K.H:598-615): Hi_k = h_k(T) * 1e-4 (J/kg; h_k = ((h_k/RT) R/W_k) T erg/g),
+   HR = 0 - Hi_0 I_R,0 - Hi_1 I_R,1 - ... in species order. Component-major I_R. */
+void orc_heat_release(int id, long ncells, const double* T, const double* IR, double* HR) {
+  const int N = MECHS[id]->nsp;
+  for (long c = 0; c < ncells; ++c) {
+    const double t = T[c], invT = 1.0 / t;
+    double hr = 0.0;
+    for (int k = 0; k < N; ++k) {
+      const sp_t* p = &SP[id][k];
+      double cpR, hRT;
+      nasa_hcp(p, t, invT, &cpR, &hRT);
+      hr = hr - (((hRT * p->RW) * t) * 0.0001) * IR[(long)k * ncells + c];
+    }
+    HR[c] = hr;
   }
 }
 

@@ -87,6 +87,10 @@ int orc_hy2t(int id, double h, const double* Y, double* T);
 /* reactionRateRhoY on ncells cells, MKS, component-major arrays; Tout nullable */
 void orc_rate_rhoY(int id, long ncells, const double* rY, const double* rH, const double* T0, double* out,
                    double* Tout);
+/* setTemperature / floorSpecies / getHeatRelease per cell, MKS, component-major species arrays */
+void orc_set_temperature(int id, long ncells, const double* rho, const double* rY, const double* rH, double* T);
+void orc_floor_species(int id, long ncells, double* rY, double* rho, double* rH, const double* T);
+void orc_heat_release(int id, long ncells, const double* T, const double* IR, double* HR);
 
 /* Reactor right-hand side on y = (rhoY_1..N, T) (CGS) with rho h(t) = rhoh0 + t*rhohsrc. */
 typedef struct {

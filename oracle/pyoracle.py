@@ -63,6 +63,9 @@ def lib():
         L.orc_mech_weights.argtypes = [C.c_int, dp]
         L.orc_rates_fr.argtypes = [C.c_int, dp, C.c_double, dp, dp]
         L.orc_rate_rhoY.argtypes = [C.c_int, C.c_long, dp, dp, dp, dp, dp]
+        L.orc_set_temperature.argtypes = [C.c_int, C.c_long, dp, dp, dp, dp]
+        L.orc_floor_species.argtypes = [C.c_int, C.c_long, dp, dp, dp, dp]
+        L.orc_heat_release.argtypes = [C.c_int, C.c_long, dp, dp, dp]
         L.orc_exp.argtypes = [C.c_double]
         L.orc_exp.restype = C.c_double
         L.orc_log.argtypes = [C.c_double]
@@ -120,6 +123,28 @@ class Mech:
         out, T = np.zeros((self.nsp, nc)), np.zeros(nc)
         lib().orc_rate_rhoY(self.id, nc, _p(rY), _p(rH), _p(T0), _p(out), _p(T))
         return out, T
+
+    def set_temperature(self, rho, rY, rH, T):
+        """setTemperature on cells (MKS; rY (N, ncells)); returns the new T."""
+        rho, rY, rH = (np.ascontiguousarray(x, dtype=np.float64) for x in (rho, rY, rH))
+        T = np.array(T, dtype=np.float64, copy=True)
+        lib().orc_set_temperature(self.id, T.size, _p(rho), _p(rY), _p(rH), _p(T))
+        return T
+
+    def floor_species(self, rY, T):
+        """floorSpecies on cells (MKS); returns (clipped rY, rho, rH)."""
+        rY = np.array(rY, dtype=np.float64, order="C", copy=True)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        rho, rH = np.zeros(T.size), np.zeros(T.size)
+        lib().orc_floor_species(self.id, T.size, _p(rY), _p(rho), _p(rH), _p(T))
+        return rY, rho, rH
+
+    def heat_release(self, T, IR):
+        """getHeatRelease on cells (MKS; IR (N, ncells)); returns HR (W m^-3)."""
+        T, IR = (np.ascontiguousarray(x, dtype=np.float64) for x in (T, IR))
+        HR = np.zeros(T.size)
+        lib().orc_heat_release(self.id, T.size, _p(T), _p(IR), _p(HR))
+        return HR
 
     def hy2t(self, h, Y, Tguess):
         Y = np.ascontiguousarray(Y, dtype=np.float64)

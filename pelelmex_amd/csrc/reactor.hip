@@ -18,6 +18,7 @@
 
 #include "../../include/pele_reactor_capi.h"
 #include "surrogate_kernel.h"
+#include "eos_kernel.h"
 #include "mech/drm19.h"
 #include "mech/lidryer.h"
 
@@ -59,6 +60,7 @@ using RateLaunch = hipError_t (*)(const prk::RateParams&, int blocks, int block,
 using TrustLaunch = hipError_t (*)(const prk::TrustParams&, int blocks, int block, size_t shmem, hipStream_t);
 using FeatLaunch = hipError_t (*)(const prk::FeatParams&, hipStream_t);
 using PrepLaunch = hipError_t (*)(const prk::PrepParams&, int blocks, int block, size_t shmem, hipStream_t);
+using EosLaunch = hipError_t (*)(int mode, const prk::EosParams&, int blocks, int block, size_t shmem, hipStream_t);
 
 template <int N, int R, int G, int W>
 hipError_t launch_t(const prk::KParams& P, int blocks, int block, size_t shmem, hipStream_t st) {
@@ -97,6 +99,17 @@ hipError_t prep_t(const prk::PrepParams& P, int blocks, int block, size_t shmem,
   return hipGetLastError();
 }
 
+template <int N, int R, int G>
+hipError_t eos_t(int mode, const prk::EosParams& P, int blocks, int block, size_t shmem, hipStream_t st) {
+  if (mode == prk::EOS_SET_T)
+    hipLaunchKernelGGL((prk::eos_kernel<N, R, G, prk::EOS_SET_T>), dim3(blocks), dim3(block), shmem, st, P);
+  else if (mode == prk::EOS_FLOOR)
+    hipLaunchKernelGGL((prk::eos_kernel<N, R, G, prk::EOS_FLOOR>), dim3(blocks), dim3(block), shmem, st, P);
+  else
+    hipLaunchKernelGGL((prk::eos_kernel<N, R, G, prk::EOS_HEAT_RELEASE>), dim3(blocks), dim3(block), shmem, st, P);
+  return hipGetLastError();
+}
+
 template <int N>
 hipError_t feat_t(const prk::FeatParams& P, hipStream_t st) {
   if (P.ncells <= 0) return hipSuccess;
@@ -128,11 +141,12 @@ struct Variant {
   TrustLaunch trust;
   FeatLaunch feat;
   PrepLaunch prep;
+  EosLaunch eos;
 };
 
 #define PR_VARIANT(NAME, NS, NR, G, W) \
   {NAME, G, W, &launch_t<NS, NR, G, W>, &eval_t<NS, NR, G>, &occ_t<NS, NR, G, W>, scratch_bytes<NS, NR, G>(), \
-   prk::Cell<NS, NR, G>::gslot_doubles * 8, &rate_t<NS, NR, G>, &trust_t<NS, NR, G>, &feat_t<NS>, &prep_t<NS, NR, G>}
+   prk::Cell<NS, NR, G>::gslot_doubles * 8, &rate_t<NS, NR, G>, &trust_t<NS, NR, G>, &feat_t<NS>, &prep_t<NS, NR, G>, &eos_t<NS, NR, G>}
 
 // drm19 at 32 lanes runs 3 waves per SIMD (24 cells in a 768-thread block, 154 KB of LDS per
 // CU): some registers spill, yet it is 18% faster than 2 waves with none (the integrator is
@@ -1223,6 +1237,78 @@ int pr_reaction_rate_box(pr_handle* h, const int lo[3], const int hi[3], const p
   HIPCHK(h->var->rate(P, blocks, block, shmem, st));
   HIPCHK(hipStreamSynchronize(st));
   return 0;
+}
+
+// Launches one of the EOS box kernels (one group per cell, grid-stride) and synchronizes.
+static int eos_launch(pr_handle* h, int mode, const int lo[3], const int hi[3], prk::EosParams& P, void* stream) {
+  const long nx = (long)hi[0] - lo[0] + 1, ny = (long)hi[1] - lo[1] + 1, nz = (long)hi[2] - lo[2] + 1;
+  if (nx <= 0 || ny <= 0 || nz <= 0) return fail(PR_ERR_ARG, "empty box");
+  const long ncells = nx * ny * nz;
+  if (ncells > 0x7fffffffL) return fail(PR_ERR_ARG, "box too large");
+  HIPCHK(hipSetDevice(h->device));
+  P.lo0 = lo[0];
+  P.lo1 = lo[1];
+  P.lo2 = lo[2];
+  P.nx = (int)nx;
+  P.ny = (int)ny;
+  P.ncells = ncells;
+  P.blob = h->d_blob;
+  P.blob_bytes = h->blob_bytes;
+  P.neff = h->mech->neff;
+  P.ninc = h->mech->ninc;
+  const int G = h->var->G;
+  const int block = pick_block(h);
+  const size_t shmem = (size_t)h->blob_bytes + (size_t)(block / G) * h->var->scratch;
+  int dev_cus = 0;
+  HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, h->device));
+  const long want = (ncells + (block / G) - 1) / (block / G);
+  const long cap = 8L * dev_cus;
+  const int blocks = (int)(want < cap ? want : cap);
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(h->var->eos(mode, P, blocks, block, shmem, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int pr_set_temperature_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rho,
+                           const pr_array4* rY, const pr_array4* rH, const pr_array4* T, void* stream) {
+  if (!h || !lo || !hi) return fail(PR_ERR_ARG, "NULL argument");
+  const int N = h->mech->nsp;
+  if (!arr_ok(rho, 1) || !arr_ok(rY, N) || !arr_ok(rH, 1) || !arr_ok(T, 1)) return fail(PR_ERR_ARG, "missing array");
+  if (rY->ncomp && rY->ncomp < N) return fail(PR_ERR_ARG, "rY needs NUM_SPECIES components");
+  prk::EosParams P{};
+  P.rho = to_arr(rho);
+  P.rY = to_arr(rY);
+  P.rH = to_arr(rH);
+  P.T = to_arr(T);
+  return eos_launch(h, prk::EOS_SET_T, lo, hi, P, stream);
+}
+
+int pr_floor_species_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY,
+                         const pr_array4* rho, const pr_array4* rH, const pr_array4* T, void* stream) {
+  if (!h || !lo || !hi) return fail(PR_ERR_ARG, "NULL argument");
+  const int N = h->mech->nsp;
+  if (!arr_ok(rY, N) || !arr_ok(rho, 1) || !arr_ok(rH, 1) || !arr_ok(T, 1)) return fail(PR_ERR_ARG, "missing array");
+  if (rY->ncomp && rY->ncomp < N) return fail(PR_ERR_ARG, "rY needs NUM_SPECIES components");
+  prk::EosParams P{};
+  P.rY = to_arr(rY);
+  P.rho = to_arr(rho);
+  P.rH = to_arr(rH);
+  P.T = to_arr(T);
+  return eos_launch(h, prk::EOS_FLOOR, lo, hi, P, stream);
+}
+
+int pr_heat_release_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* T,
+                        const pr_array4* IR, const pr_array4* HR, void* stream) {
+  if (!h || !lo || !hi) return fail(PR_ERR_ARG, "NULL argument");
+  const int N = h->mech->nsp;
+  if (!arr_ok(T, 1) || !arr_ok(IR, N) || !arr_ok(HR, 1)) return fail(PR_ERR_ARG, "missing array");
+  if (IR->ncomp && IR->ncomp < N) return fail(PR_ERR_ARG, "I_R needs NUM_SPECIES components");
+  prk::EosParams P{};
+  P.T = to_arr(T);
+  P.IR = to_arr(IR);
+  P.HR = to_arr(HR);
+  return eos_launch(h, prk::EOS_HEAT_RELEASE, lo, hi, P, stream);
 }
 
 int pr_surrogate_load(pr_handle* h, int nlayers, const int* dims, const float* params, const float* xnorm,

@@ -23,7 +23,8 @@ LIB_PATH = os.path.join(_HERE, "libpelelmex_amd.so")
 _LIB = None
 
 EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_react_batch", "pr_react_batch_enqueue", "pr_collect", "pr_rhs_batch", "pr_jac_batch",
-           "pr_reaction_rate_box", "pr_surrogate_load", "pr_surrogate_features", "pr_mlp_gemm",
+           "pr_reaction_rate_box", "pr_set_temperature_box", "pr_floor_species_box",
+           "pr_heat_release_box", "pr_surrogate_load", "pr_surrogate_features", "pr_mlp_gemm",
            "pr_mlp_gelu_backward", "pr_mlp_column_sum", "pr_mlp_mse", "pr_mlp_adam", "pr_mlp_to_bf16",
            "pr_get_stats", "pr_num_species", "pr_destroy", "pr_last_error"]
 
@@ -85,6 +86,12 @@ def lib():
                                            C.POINTER(Array4), C.POINTER(Array4), C.POINTER(Array4),
                                            C.POINTER(IArray4), C.c_void_p]
         L.pr_reaction_rate_box.restype = C.c_int
+        a4 = C.POINTER(Array4)
+        L.pr_set_temperature_box.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), a4, a4, a4, a4,
+                                             C.c_void_p]
+        L.pr_floor_species_box.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), a4, a4, a4, a4,
+                                           C.c_void_p]
+        L.pr_heat_release_box.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), a4, a4, a4, C.c_void_p]
         vp, i64, fp = C.c_void_p, C.c_int64, C.POINTER(C.c_float)
         L.pr_surrogate_load.argtypes = [vp, C.c_int, C.POINTER(C.c_int), fp, fp, fp]
         L.pr_surrogate_features.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
@@ -249,6 +256,38 @@ class Reactor:
         if rc != 0:
             raise ReactorError(f"pr_reaction_rate_box failed ({rc}): {lib().pr_last_error().decode()}")
         return rYdot
+
+    def _eos(self, fn, arrays, shape, stream):
+        import torch
+        nz, ny, nx = shape
+        lo = (C.c_int * 3)(0, 0, 0)
+        hi = (C.c_int * 3)(nx - 1, ny - 1, nz - 1)
+        st = stream if stream is not None else torch.cuda.current_stream()
+        rc = getattr(lib(), fn)(self._h, lo, hi, *[C.byref(x) for x in arrays], C.c_void_p(st.cuda_stream))
+        if rc != 0:
+            raise ReactorError(f"{fn} failed ({rc}): {lib().pr_last_error().decode()}")
+
+    def set_temperature(self, rho, rY, rH, T, stream=None):
+        """setTemperature over a box of device tensors (MKS: rY [N, nz, ny, nx], rho, rH and T
+        [nz, ny, nx]); T is updated in place (HY2T from its stored value)."""
+        self._eos("pr_set_temperature_box", [_array4(rho, 1), _array4(rY, self.nsp), _array4(rH, 1), _array4(T, 1)],
+                  T.shape, stream)
+        return T
+
+    def floor_species(self, rY, rho, rH, T, stream=None):
+        """floorSpecies over a box: clips rY to [0, DBL_MAX] in place and rewrites rho and rH."""
+        self._eos("pr_floor_species_box", [_array4(rY, self.nsp), _array4(rho, 1), _array4(rH, 1), _array4(T, 1)],
+                  T.shape, stream)
+        return rY, rho, rH
+
+    def heat_release(self, T, IR, HR=None, stream=None):
+        """getHeatRelease over a box: HR = -sum_k h_k(T) IR_k (W m^-3) from T [nz, ny, nx] and
+        IR [N, nz, ny, nx] (kg m^-3 s^-1)."""
+        import torch
+        if HR is None:
+            HR = torch.empty_like(T)
+        self._eos("pr_heat_release_box", [_array4(T, 1), _array4(IR, self.nsp), _array4(HR, 1)], T.shape, stream)
+        return HR
 
     def load_surrogate(self, dims, params, xnorm, ynorm):
         """pr_surrogate_load: dims (layer widths), params (fp32 W0, b0, W1, b1, ... flattened),

@@ -383,3 +383,45 @@ def test_array4_views_inside_ghosted_fabs(gpu):
     torch.cuda.synchronize()
     _assert_bits("rYdot", out.cpu()[inner].numpy(), want.cpu().numpy())
     r.close()
+
+
+@pytest.mark.parametrize("mech_box,lanes", [("h2", 16), ("h2", 8), ("ch4", 32), ("ch4", 16), ("ch4", 64)])
+def test_eos_box_kernels_bits(gpu, mech_box, lanes):
+    """setTemperature, floorSpecies and getHeatRelease over a box (MKS Array4 views), bit for bit
+    against the oracle for every lanes-per-cell variant; floorSpecies' input carries negative,
+    -0 and NaN species densities (clipped to +0 and DBL_MAX as std::min / std::max do)."""
+    import torch
+    from oracle import pyoracle as po
+    from pelelmex_amd.reactor import Reactor
+    box = W.h2_air_box(nx=16, ny=8, nz=4) if mech_box == "h2" else W.ch4_air_flame_box(nx=16, ny=8, nz=4)
+    nx, ny, nz = box["dims"]
+    nc = nx * ny * nz
+    m = po.Mech(box["mech"])
+    N = m.nsp
+    rY, rH, T = box["rY"] * 1e3, box["rH"] * 0.1, box["T"].copy()
+    rho = rY.sum(0)
+    T0 = T + 40.0 * np.cos(np.arange(nc))
+    r = Reactor.create("ReactorCvode", box["mech"], dict(KEYS, **{"hipbdf.lanes_per_cell": lanes}))
+    r.init(2, 1)
+    dev = lambda a, shp: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64).reshape(shp).to(gpu)
+    s3, sN = (nz, ny, nx), (N, nz, ny, nx)
+    # setTemperature
+    Tg = dev(T0, s3)
+    r.set_temperature(dev(rho, s3), dev(rY, sN), dev(rH, s3), Tg)
+    _assert_bits("T", Tg.cpu().numpy().reshape(-1), m.set_temperature(rho, rY, rH, T0))
+    # floorSpecies
+    rYn = rY.copy()
+    rYn[1, ::3] = -1e-9
+    rYn[2, 1::4] = -0.0
+    rYn[0, 5] = np.nan
+    ref_rY, ref_rho, ref_rH = m.floor_species(rYn, T)
+    gY, grho, grH = dev(rYn, sN), torch.zeros(s3, dtype=torch.float64, device=gpu), torch.zeros(s3, dtype=torch.float64, device=gpu)
+    r.floor_species(gY, grho, grH, dev(T, s3))
+    _assert_bits("rY", gY.cpu().numpy().reshape(N, -1), ref_rY)
+    _assert_bits("rho", grho.cpu().numpy().reshape(-1), ref_rho)
+    _assert_bits("rH", grH.cpu().numpy().reshape(-1), ref_rH)
+    # getHeatRelease on the reactionRateRhoY rates
+    IR, _ = m.rate_rhoY(rY, rH, T)
+    HR = r.heat_release(dev(T, s3), dev(IR, sN))
+    _assert_bits("HR", HR.cpu().numpy().reshape(-1), m.heat_release(T, IR))
+    r.close()

@@ -316,3 +316,50 @@ def test_reaction_rate_rhoY_pins(name):
         cell = m.cell(box["rH"][c], 0.0, np.zeros(m.nsp))
         f = m.rhs(cell, 0.0, np.append(box["rY"][:, c], T[c]), frozen=True)
         np.testing.assert_allclose(out[:, c] * 1e-3, f[:-1], rtol=1e-12, atol=1e-12 * np.abs(f[:-1]).max())
+
+
+@pytest.mark.parametrize("name", ["h2", "ch4"])
+def test_eos_box_pins(name):
+    """setTemperature / floorSpecies / getHeatRelease restatements (Source/PeleLMeX_Eos.cpp:218-240,
+    Source/PeleLMeX_Utils.cpp:649-690, Source/PeleLMeX_Reactions.cpp:477-506), MKS. Against the
+    NASA-7 fits evaluated independently (plain polynomials of the raw coefficients): floorSpecies
+    clips as std::min / std::max do, its density is the species sum and its rhoH the mixture
+    enthalpy times 1e-4 times rho; setTemperature inverts it from a temperature guess 50 K away;
+    the heat release is -sum_k h_k(T) I_R,k in J/kg times kg m^-3 s^-1, with I_R the
+    reactionRateRhoY rates. A wrong unit factor, sign, clipping branch or density fails."""
+    box = W.h2_air_box(nx=8, ny=2, nz=2) if name == "h2" else W.ch4_air_flame_box(nx=16, ny=2, nz=2)
+    m = po.Mech(box["mech"])
+    mech = load(box["mech"])
+    nc = box["T"].size
+    rY, rH, T = box["rY"] * 1e3, box["rH"] * 0.1, box["T"]
+    # floorSpecies
+    rYn = rY.copy()
+    rYn[1, ::3] = -1e-9
+    rYn[2, 1::4] = -0.0
+    rYn[0, 5] = np.nan
+    rYf, rho, rHf = m.floor_species(rYn, T)
+    expect = rYn.copy()
+    expect[1, ::3] = 0.0
+    expect[2, 1::4] = 0.0
+    expect[0, 5] = np.finfo(np.float64).max
+    np.testing.assert_array_equal(rYf, expect)
+    assert not np.any(np.signbit(rYf))
+    ok = np.ones(nc, dtype=bool)
+    ok[5] = False
+    np.testing.assert_allclose(rho[ok], rYf[:, ok].sum(0), rtol=1e-15)
+    Y = (rYf[:, ok] / rho[ok]).T
+    np.testing.assert_allclose(rHf[ok], mech.h_mass(T[ok], Y) * 1e-4 * rho[ok], rtol=1e-12,
+                               atol=1e-12 * np.abs(rHf[ok]).max())
+    # setTemperature from a guess 50 K away recovers T
+    T2 = m.set_temperature(rho[ok], rYf[:, ok], rHf[ok], T[ok] + 50.0 * np.sin(np.arange(ok.sum()) + 1.0))
+    np.testing.assert_allclose(T2, T[ok], rtol=1e-10)
+    # getHeatRelease
+    IR, _ = m.rate_rhoY(rY, rH, T)
+    HR = m.heat_release(T, IR)
+    hk = mech.h_RT(T) * RU * T[:, None] / mech.W * 1e-4           # J/kg, [nc, N]
+    ref = -np.sum(hk * IR.T, axis=1)
+    np.testing.assert_allclose(HR, ref, rtol=1e-10, atol=1e-10 * np.abs(hk * IR.T).sum(1).max())
+    assert np.any(HR > 0)                                         # the flame releases heat
+    single = np.zeros_like(IR)
+    single[3] = 1.0
+    np.testing.assert_allclose(m.heat_release(T, single), -hk[:, 3], rtol=1e-13)
