@@ -210,6 +210,8 @@ def main():
                                "1 atm, 5% Gaussian sources",
                    "mechanism": "drm19", "cells": ncells, "dt_react": box["dt"], "rtol": 1e-10, "atol": 1e-10,
                    "max_order": args.max_order, "lanes_per_cell": st["lanes_per_cell"], "seed": 2002,
+                   "cell_order": "hipbdf.cost_order = 1: FC carried from the previous step (as PeleLMeX's functC) "
+                                 "orders the work queue; the first (warm-up) step starts from FC = 0",
                    "path": "learned map (BF16 MFMA) + FP64 trust check + GPU BDF fallback" if sur
                    else "plain GPU BDF"},
         "bdf_value": bdf_value,

@@ -954,6 +954,68 @@ __global__ void gather_sample_kernel(BoxDesc B, int N, long S, double* out) {
   out[(2L * N + 3) * S + a] = 0.0;
 }
 
+// Cost order of a box's cells (or of a list of them) from the input FC (right-hand-side
+// evaluations of the previous advance, PeleLMeX's chemistry cost measure, Source/PeleLMeX.H:209):
+// 128 buckets, 4 per octave of FC + 1, the most expensive first, so the persistent kernel starts
+// the stiff cells early and its tail is short. order[] lists the cell indices bucket by bucket
+// (arbitrary order within a bucket). hist holds 2 x COST_BUCKETS counters, zero on entry.
+constexpr int COST_BUCKETS = 128;
+
+__device__ __forceinline__ int cost_bucket(double fc) {
+  const double f = (fc > 0.0) ? (fc < 2.0e9 ? fc : 2.0e9) : 0.0;
+  const unsigned u = (unsigned)f + 1u;
+  const int msb = 31 - __builtin_clz(u);
+  const int frac = (msb >= 2) ? (int)((u >> (msb - 2)) & 3u) : (int)((u << (2 - msb)) & 3u);
+  const int key = msb * 4 + frac;  // < 128
+  return COST_BUCKETS - 1 - key;
+}
+
+// The cells to order: list[0 .. min(*nlist, nmax)) when list is set, else the whole box.
+__device__ __forceinline__ long cost_count(const BoxDesc& B, const int* list, const unsigned int* nlist, long nmax) {
+  if (!list) return B.ncells;
+  const long n = (long)*nlist;
+  return n < nmax ? n : nmax;
+}
+
+__global__ void cost_hist_kernel(BoxDesc B, const int* list, const unsigned int* nlist, long nmax, unsigned int* hist) {
+  const long n = cost_count(B, list, nlist, nmax);
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x) {
+    const long c = list ? (long)list[x] : x;
+    const unsigned u = (unsigned)c;
+    const int i = B.lo0 + (int)(u % (unsigned)B.nx);
+    const int j = B.lo1 + (int)((u / (unsigned)B.nx) % (unsigned)B.ny);
+    const int k = B.lo2 + (int)(u / ((unsigned)B.nx * (unsigned)B.ny));
+    atomicAdd(&hist[cost_bucket(B.FC.p[aoff(B.FC, i, j, k)])], 1u);
+  }
+}
+
+// exclusive prefix sums of the bucket counts into the cursors (one thread)
+__global__ void cost_scan_kernel(unsigned int* hist) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    unsigned int a = 0;
+    for (int b = 0; b < COST_BUCKETS; ++b) {
+      hist[COST_BUCKETS + b] = a;
+      a += hist[b];
+    }
+  }
+}
+
+__global__ void cost_scatter_kernel(BoxDesc B, const int* list, const unsigned int* nlist, long nmax,
+                                    unsigned int* hist, int* order) {
+  const long n = cost_count(B, list, nlist, nmax);
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x) {
+    const long c = list ? (long)list[x] : x;
+    const unsigned u = (unsigned)c;
+    const int i = B.lo0 + (int)(u % (unsigned)B.nx);
+    const int j = B.lo1 + (int)((u / (unsigned)B.nx) % (unsigned)B.ny);
+    const int k = B.lo2 + (int)(u / ((unsigned)B.nx * (unsigned)B.ny));
+    const int b = cost_bucket(B.FC.p[aoff(B.FC, i, j, k)]);
+    // one bucket holding every cell (e.g. FC all zero on a first advance): the given order
+    const unsigned pos = (hist[b] == (unsigned)n) ? (unsigned)x : atomicAdd(&hist[COST_BUCKETS + b], 1u);
+    order[pos] = (int)c;
+  }
+}
+
 __global__ void zero_counters(unsigned int* queue, unsigned long long* stats, unsigned int* nfail) {
   int t = threadIdx.x;
   if (t < 8) stats[t] = 0ull;

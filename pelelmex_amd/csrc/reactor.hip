@@ -247,6 +247,7 @@ struct Keys {
   std::string solve_type = "GMRES";
   int clean_init_massfrac = 0, analytical_jacobian = 1, verbose = 0;
   int lanes_per_cell = 0, block = 0, autotune = 0;
+  int cost_order = 1;  // hand cells out by decreasing input FC (PeleLMeX passes the previous advance's)
   // learned map in front of the BDF (used once weights are loaded with pr_surrogate_load)
   int surr_enable = 1;
   double surr_trust_rtol = 1e-3, surr_trust_atol = 1e-9, surr_elem_tol = 0.05, surr_audit_frac = 0.01;
@@ -312,6 +313,9 @@ int parse_keys(const char* text, Keys& K) {
     } else if (k == "hipbdf.lanes_per_cell") {
       if (!num(d)) return fail(PR_ERR_KEY, "bad hipbdf.lanes_per_cell");
       K.lanes_per_cell = (int)d;
+    } else if (k == "hipbdf.cost_order") {
+      if (!num(d) || (d != 0 && d != 1)) return fail(PR_ERR_KEY, "hipbdf.cost_order must be 0 or 1");
+      K.cost_order = (int)d;
     } else if (k == "hipbdf.autotune") {
       if (!num(d) || (d != 0 && d != 1)) return fail(PR_ERR_KEY, "hipbdf.autotune must be 0 or 1");
       K.autotune = (int)d;
@@ -386,6 +390,9 @@ struct pr_handle {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   Surrogate* surr = nullptr;
   bool tuned = false;        // hipbdf.autotune has chosen var / block
+  int* d_order = nullptr;     // hipbdf.cost_order: cell order of the box (cells, capacity order_cap)
+  unsigned int* d_hist = nullptr;  // its bucket counts and cursors (2 x COST_BUCKETS)
+  long order_cap = 0;
   void* d_boxtab = nullptr;  // device box table (descriptors, then cell prefix sums)
   void* h_boxtab = nullptr;  // pinned staging copy of it
   size_t boxtab_bytes = 0;
@@ -942,6 +949,28 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   P.boxes = dboxes;
   P.box_start = dstart;
   P.nbox = 1;
+  if (h->keys.cost_order) {
+    // hipbdf.cost_order: the cells (all, or the learned map's rejects) handed out by decreasing
+    // input FC, the previous advance's cost
+    if (h->order_cap < ncells) {
+      if (h->d_order) (void)hipFree(h->d_order);
+      h->d_order = nullptr;
+      h->order_cap = 0;
+      HIPCHK(hipMalloc(&h->d_order, sizeof(int) * (size_t)ncells));
+      h->order_cap = ncells;
+    }
+    if (!h->d_hist) HIPCHK(hipMalloc(&h->d_hist, sizeof(unsigned int) * 2 * prk::COST_BUCKETS));
+    HIPCHK(hipMemsetAsync(h->d_hist, 0, sizeof(unsigned int) * 2 * prk::COST_BUCKETS, st));
+    const int* list = P.cell_list;
+    const unsigned int* nlist = P.ncells_dev;
+    const unsigned grid = (unsigned)((ncells + 255) / 256 < 4096 ? (ncells + 255) / 256 : 4096);
+    hipLaunchKernelGGL(prk::cost_hist_kernel, dim3(grid), dim3(256), 0, st, P.one, list, nlist, ncells, h->d_hist);
+    hipLaunchKernelGGL(prk::cost_scan_kernel, dim3(1), dim3(64), 0, st, h->d_hist);
+    hipLaunchKernelGGL(prk::cost_scatter_kernel, dim3(grid), dim3(256), 0, st, P.one, list, nlist, ncells, h->d_hist,
+                       h->d_order);
+    HIPCHK(hipGetLastError());
+    P.cell_list = h->d_order;  // P.ncells_dev (the reject count) is unchanged
+  }
   {
     int rc = bdf_launch(h, P, ncells, st);
     if (rc) return rc;
@@ -1529,6 +1558,8 @@ void pr_destroy(pr_handle* h) {
   if (h->ev3) (void)hipEventDestroy(h->ev3);
   if (h->d_boxtab) (void)hipFree(h->d_boxtab);
   if (h->h_boxtab) (void)hipHostFree(h->h_boxtab);
+  if (h->d_order) (void)hipFree(h->d_order);
+  if (h->d_hist) (void)hipFree(h->d_hist);
   if (h->surr) {
     for (void* q : h->surr->owned) (void)hipFree(q);
     delete h->surr;

@@ -425,3 +425,34 @@ def test_eos_box_kernels_bits(gpu, mech_box, lanes):
     HR = r.heat_release(dev(T, s3), dev(IR, sN))
     _assert_bits("HR", HR.cpu().numpy().reshape(-1), m.heat_release(T, IR))
     r.close()
+
+
+def test_cost_order_changes_nothing_but_the_order(gpu):
+    """hipbdf.cost_order = 1 hands cells out by decreasing input FC (the previous advance's
+    right-hand-side count): run on the FC of a first run, the second run's results and counters
+    are bit-identical to the first's."""
+    import torch
+    from pelelmex_amd.reactor import Reactor, box_tensors
+    box = W.ch4_air_flame_box(nx=64, ny=8, nz=4, seed=43)
+    box["mask"][::11] = -1
+    keys = dict(KEYS, **{"cvode.max_order": 4, "hipbdf.lanes_per_cell": 32})
+    outs = []
+    fc_prev = None
+    for order in (0, 1):
+        r = Reactor.create("ReactorCvode", box["mech"], dict(keys, **{"hipbdf.cost_order": order}))
+        r.init(2, 1)
+        t = box_tensors(box, gpu)
+        if fc_prev is not None:
+            t["FC"].copy_(fc_prev)
+        nf, _, _ = r.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], t["mask"], box["dt"], 0.0)
+        torch.cuda.synchronize()
+        st = r.stats()
+        r.close()
+        fc_prev = t["FC"].clone()
+        outs.append(({k: t[k].cpu().numpy() for k in ("rY", "T", "rH", "FC")}, nf, st))
+    assert float(fc_prev.max()) > 0.0
+    for k in ("rY", "T", "rH", "FC"):
+        _assert_bits(k, outs[1][0][k].reshape(-1), outs[0][0][k].reshape(-1))
+    assert outs[0][1] == outs[1][1] == 0
+    for k in ("cells", "nst", "nfe", "nje", "nsetups", "nni", "netf", "ncfn"):
+        assert outs[0][2][k] == outs[1][2][k], k
