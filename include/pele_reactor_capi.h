@@ -28,6 +28,7 @@
  *   pr_reaction_rate_box  reactionRateRhoY over a box (Source/PeleLMeX_K.H:848-889), as
  *                    PeleLM::computeInstantaneousReactionRate runs it per box
  *                    (Source/PeleLMeX_Reactions.cpp:399-435)
+ *   pr_box_costs     PeleLM::computeCosts for the chemistry BoxArray (Source/PeleLMeX_Regrid.cpp:558-606)
  *   pr_set_temperature_box  PeleLM::setTemperature per box (Source/PeleLMeX_Eos.cpp:218-240,
  *                    getTfromHY Source/PeleLMeX_K.H:700-723)
  *   pr_floor_species_box    PeleLM::floorSpecies per box (Source/PeleLMeX_Utils.cpp:649-690)
@@ -136,6 +137,14 @@ int pr_jac_batch(pr_handle* h, int64_t ncells, const double* y, const double* rY
 int pr_reaction_rate_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY,
                          const pr_array4* rH, const pr_array4* T, const pr_array4* rYdot,
                          const pr_iarray4* mask, void* stream);
+/* Chemistry cost per box from the reactor's work counters (FC = right-hand-side evaluations per
+   cell), as PeleLM::computeCosts does for the chemistry BoxArray before its knapsack / SFC
+   distribution (Source/PeleLMeX_Regrid.cpp:558-606): method 0 Ncell (cells of the box), 1
+   ChemFunctCallAvg (sum of FC / cells), 2 ChemFunctCallMax, 3 ChemFunctCallSum. Box b spans
+   lo[3b..3b+2] .. hi[3b..3b+2]; FC[b] is its (device) FC view (unused for method 0); costs is a
+   host array of nbox doubles. Waits for `stream`. */
+int pr_box_costs(pr_handle* h, int nbox, const int* lo, const int* hi, const pr_array4* FC, int method,
+                 double* costs, void* stream);
 /* Equation-of-state box kernels around the chemistry, MKS Array4 views as PeleLMeX's state
    (rho kg m^-3, rY kg m^-3, rH J m^-3, T K), one enqueue on `stream` then a wait for it.
    pr_set_temperature_box (setTemperature, Source/PeleLMeX_Eos.cpp:218-240 with getTfromHY,

@@ -1016,6 +1016,49 @@ __global__ void cost_scatter_kernel(BoxDesc B, const int* list, const unsigned i
   }
 }
 
+// Per-box chemistry cost from the FC arrays (PeleLM::computeCosts, Source/PeleLMeX_Regrid.cpp:558-606):
+// one block per box, the sum and the maximum of FC over the box's cells (a tree sum: exact for the
+// whole-number FC the reactor writes), then the method's figure.
+enum { COST_NCELL = 0, COST_FC_AVG = 1, COST_FC_MAX = 2, COST_FC_SUM = 3 };
+
+struct CostBox {
+  int lo0, lo1, lo2, nx, ny;
+  long ncells;
+  Arr FC;
+};
+
+__global__ void __launch_bounds__(256) box_cost_kernel(const CostBox* boxes, int method, double* out) {
+  __shared__ double ssum[256], smax[256];
+  const CostBox B = boxes[blockIdx.x];
+  double sum = 0.0, mx = -DBL_MAX;
+  for (long c = threadIdx.x; c < B.ncells; c += blockDim.x) {
+    const unsigned u = (unsigned)c;
+    const int i = B.lo0 + (int)(u % (unsigned)B.nx);
+    const int j = B.lo1 + (int)((u / (unsigned)B.nx) % (unsigned)B.ny);
+    const int k = B.lo2 + (int)(u / ((unsigned)B.nx * (unsigned)B.ny));
+    const double v = B.FC.p[aoff(B.FC, i, j, k)];
+    sum += v;
+    mx = v > mx ? v : mx;
+  }
+  ssum[threadIdx.x] = sum;
+  smax[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      ssum[threadIdx.x] += ssum[threadIdx.x + w];
+      smax[threadIdx.x] = smax[threadIdx.x + w] > smax[threadIdx.x] ? smax[threadIdx.x + w] : smax[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double n = (double)B.ncells;
+    out[blockIdx.x] = (method == COST_NCELL) ? n
+                      : (method == COST_FC_AVG) ? ssum[0] / n
+                      : (method == COST_FC_MAX) ? smax[0]
+                                                : ssum[0];
+  }
+}
+
 __global__ void zero_counters(unsigned int* queue, unsigned long long* stats, unsigned int* nfail) {
   int t = threadIdx.x;
   if (t < 8) stats[t] = 0ull;

@@ -1299,6 +1299,49 @@ static int eos_launch(pr_handle* h, int mode, const int lo[3], const int hi[3], 
   return 0;
 }
 
+int pr_box_costs(pr_handle* h, int nbox, const int* lo, const int* hi, const pr_array4* FC, int method,
+                 double* costs, void* stream) {
+  if (!h || nbox < 0 || (nbox > 0 && (!lo || !hi || !FC || !costs))) return fail(PR_ERR_ARG, "NULL argument");
+  if (method < prk::COST_NCELL || method > prk::COST_FC_SUM) return fail(PR_ERR_ARG, "unknown cost method");
+  if (nbox == 0) return 0;
+  std::vector<prk::CostBox> bx((size_t)nbox);
+  for (int b = 0; b < nbox; ++b) {
+    const long nx = (long)hi[3 * b] - lo[3 * b] + 1, ny = (long)hi[3 * b + 1] - lo[3 * b + 1] + 1,
+               nz = (long)hi[3 * b + 2] - lo[3 * b + 2] + 1;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return fail(PR_ERR_ARG, "empty box");
+    if (nx * ny * nz > 0x7fffffffL) return fail(PR_ERR_ARG, "box too large");
+    if (method != prk::COST_NCELL && !arr_ok(&FC[b], 1)) return fail(PR_ERR_ARG, "missing FC array");
+    bx[b].lo0 = lo[3 * b];
+    bx[b].lo1 = lo[3 * b + 1];
+    bx[b].lo2 = lo[3 * b + 2];
+    bx[b].nx = (int)nx;
+    bx[b].ny = (int)ny;
+    bx[b].ncells = nx * ny * nz;
+    bx[b].FC = to_arr(&FC[b]);
+  }
+  if (method == prk::COST_NCELL) {
+    for (int b = 0; b < nbox; ++b) costs[b] = (double)bx[b].ncells;
+    return 0;
+  }
+  HIPCHK(hipSetDevice(h->device));
+  hipStream_t st = (hipStream_t)stream;
+  void* d = nullptr;
+  const size_t bytes = sizeof(prk::CostBox) * (size_t)nbox + sizeof(double) * (size_t)nbox;
+  HIPCHK(hipMalloc(&d, bytes));
+  prk::CostBox* dbx = static_cast<prk::CostBox*>(d);
+  double* dout = reinterpret_cast<double*>(static_cast<char*>(d) + sizeof(prk::CostBox) * (size_t)nbox);
+  hipError_t e = hipMemcpyAsync(dbx, bx.data(), sizeof(prk::CostBox) * (size_t)nbox, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(prk::box_cost_kernel, dim3((unsigned)nbox), dim3(256), 0, st, dbx, method, dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(costs, dout, sizeof(double) * (size_t)nbox, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(PR_ERR_HIP, std::string("pr_box_costs: ") + hipGetErrorString(e));
+  return 0;
+}
+
 int pr_set_temperature_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rho,
                            const pr_array4* rY, const pr_array4* rH, const pr_array4* T, void* stream) {
   if (!h || !lo || !hi) return fail(PR_ERR_ARG, "NULL argument");

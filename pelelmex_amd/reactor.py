@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(_HERE, "libpelelmex_amd.so")
 _LIB = None
 
 EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_react_batch", "pr_react_batch_enqueue", "pr_collect", "pr_rhs_batch", "pr_jac_batch",
-           "pr_reaction_rate_box", "pr_set_temperature_box", "pr_floor_species_box",
+           "pr_reaction_rate_box", "pr_box_costs", "pr_set_temperature_box", "pr_floor_species_box",
            "pr_heat_release_box", "pr_surrogate_load", "pr_surrogate_features", "pr_mlp_gemm",
            "pr_mlp_gelu_backward", "pr_mlp_column_sum", "pr_mlp_mse", "pr_mlp_adam", "pr_mlp_to_bf16",
            "pr_get_stats", "pr_num_species", "pr_destroy", "pr_last_error"]
@@ -87,6 +87,8 @@ def lib():
                                            C.POINTER(IArray4), C.c_void_p]
         L.pr_reaction_rate_box.restype = C.c_int
         a4 = C.POINTER(Array4)
+        L.pr_box_costs.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), a4, C.c_int,
+                                   C.POINTER(C.c_double), C.c_void_p]
         L.pr_set_temperature_box.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), a4, a4, a4, a4,
                                              C.c_void_p]
         L.pr_floor_species_box.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), a4, a4, a4, a4,
@@ -256,6 +258,24 @@ class Reactor:
         if rc != 0:
             raise ReactorError(f"pr_reaction_rate_box failed ({rc}): {lib().pr_last_error().decode()}")
         return rYdot
+
+    COST_METHODS = {"Ncell": 0, "ChemFunctCallAvg": 1, "ChemFunctCallMax": 2, "ChemFunctCallSum": 3}
+
+    def box_costs(self, fcs, method="ChemFunctCallAvg", stream=None):
+        """Chemistry cost per box (PeleLM::computeCosts) from FC tensors [nz, ny, nx] (one per box);
+        returns a numpy array."""
+        import numpy as np
+        import torch
+        n = len(fcs)
+        lo = (C.c_int * (3 * n))(*([0] * 3 * n))
+        hi = (C.c_int * (3 * n))(*[v for f in fcs for v in (f.shape[2] - 1, f.shape[1] - 1, f.shape[0] - 1)])
+        arr = (Array4 * n)(*[_array4(f, 1) for f in fcs])
+        out = (C.c_double * n)()
+        st = stream if stream is not None else torch.cuda.current_stream()
+        rc = lib().pr_box_costs(self._h, n, lo, hi, arr, self.COST_METHODS[method], out, C.c_void_p(st.cuda_stream))
+        if rc != 0:
+            raise ReactorError(f"pr_box_costs failed ({rc}): {lib().pr_last_error().decode()}")
+        return np.array(out[:], dtype=np.float64)
 
     def _eos(self, fn, arrays, shape, stream):
         import torch

@@ -456,3 +456,32 @@ def test_cost_order_changes_nothing_but_the_order(gpu):
     assert outs[0][1] == outs[1][1] == 0
     for k in ("cells", "nst", "nfe", "nje", "nsetups", "nni", "netf", "ncfn"):
         assert outs[0][2][k] == outs[1][2][k], k
+
+
+def test_box_costs(gpu):
+    """pr_box_costs (PeleLM::computeCosts for the chemistry BoxArray): Ncell, and the average,
+    maximum and sum of FC per box, exactly as numpy computes them for whole-number FC, on boxes
+    of three sizes; a reactor's own FC gives a cost that grows with the box's stiff content."""
+    import torch
+    from pelelmex_amd.reactor import Reactor, box_tensors
+    rng = np.random.default_rng(7)
+    shapes = [(4, 4, 4), (8, 4, 2), (16, 16, 16)]
+    fcs = [torch.as_tensor(rng.integers(0, 3000, size=s).astype(np.float64)).to(gpu) for s in shapes]
+    r = Reactor.create("ReactorCvode", "drm19", KEYS)
+    r.init(2, 1)
+    n = np.array([f.numel() for f in fcs], dtype=np.float64)
+    np.testing.assert_array_equal(r.box_costs(fcs, "Ncell"), n)
+    host = [f.cpu().numpy() for f in fcs]
+    np.testing.assert_array_equal(r.box_costs(fcs, "ChemFunctCallSum"), [h.sum() for h in host])
+    np.testing.assert_array_equal(r.box_costs(fcs, "ChemFunctCallMax"), [h.max() for h in host])
+    np.testing.assert_array_equal(r.box_costs(fcs, "ChemFunctCallAvg"), [h.sum() / h.size for h in host])
+    # costs from a react: the flame-front slab costs more per cell than the cold slab
+    box = W.ch4_air_flame_box(nx=64, ny=4, nz=4, seed=5)
+    t = box_tensors(box, gpu)
+    r.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], t["mask"], box["dt"], 0.0)
+    torch.cuda.synchronize()
+    fc = t["FC"].reshape(4, 4, 64)
+    cold, front = fc[:, :, :16].contiguous(), fc[:, :, 20:30].contiguous()
+    c = r.box_costs([cold, front], "ChemFunctCallAvg")
+    assert c[1] > c[0] > 0
+    r.close()
