@@ -977,7 +977,11 @@ __device__ __forceinline__ long cost_count(const BoxDesc& B, const int* list, co
   return n < nmax ? n : nmax;
 }
 
-__global__ void cost_hist_kernel(BoxDesc B, const int* list, const unsigned int* nlist, long nmax, unsigned int* hist) {
+__global__ void __launch_bounds__(256) cost_hist_kernel(BoxDesc B, const int* list, const unsigned int* nlist,
+                                                        long nmax, unsigned int* hist) {
+  __shared__ unsigned int h[COST_BUCKETS];
+  for (int b = threadIdx.x; b < COST_BUCKETS; b += blockDim.x) h[b] = 0u;
+  __syncthreads();
   const long n = cost_count(B, list, nlist, nmax);
   for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x) {
     const long c = list ? (long)list[x] : x;
@@ -985,11 +989,13 @@ __global__ void cost_hist_kernel(BoxDesc B, const int* list, const unsigned int*
     const int i = B.lo0 + (int)(u % (unsigned)B.nx);
     const int j = B.lo1 + (int)((u / (unsigned)B.nx) % (unsigned)B.ny);
     const int k = B.lo2 + (int)(u / ((unsigned)B.nx * (unsigned)B.ny));
-    atomicAdd(&hist[cost_bucket(B.FC.p[aoff(B.FC, i, j, k)])], 1u);
+    atomicAdd(&h[cost_bucket(B.FC.p[aoff(B.FC, i, j, k)])], 1u);  // block-private counts in LDS
   }
+  __syncthreads();
+  for (int b = threadIdx.x; b < COST_BUCKETS; b += blockDim.x)
+    if (h[b]) atomicAdd(&hist[b], h[b]);
 }
 
-// exclusive prefix sums of the bucket counts into the cursors (one thread)
 __global__ void cost_scan_kernel(unsigned int* hist) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     unsigned int a = 0;
@@ -1000,18 +1006,35 @@ __global__ void cost_scan_kernel(unsigned int* hist) {
   }
 }
 
-__global__ void cost_scatter_kernel(BoxDesc B, const int* list, const unsigned int* nlist, long nmax,
-                                    unsigned int* hist, int* order) {
+// Each block counts its cells per bucket in LDS, reserves one range per bucket from the global
+// cursors, and places its cells inside its ranges (one grid-stride round per launch: the grid
+// covers every cell, so each thread handles at most one).
+__global__ void __launch_bounds__(256) cost_scatter_kernel(BoxDesc B, const int* list, const unsigned int* nlist,
+                                                           long nmax, unsigned int* hist, int* order) {
+  __shared__ unsigned int cnt[COST_BUCKETS], base[COST_BUCKETS];
+  for (int b = threadIdx.x; b < COST_BUCKETS; b += blockDim.x) cnt[b] = 0u;
+  __syncthreads();
   const long n = cost_count(B, list, nlist, nmax);
-  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x) {
-    const long c = list ? (long)list[x] : x;
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long c = -1;
+  int b = 0;
+  unsigned int r = 0;
+  if (x < n) {
+    c = list ? (long)list[x] : x;
     const unsigned u = (unsigned)c;
     const int i = B.lo0 + (int)(u % (unsigned)B.nx);
     const int j = B.lo1 + (int)((u / (unsigned)B.nx) % (unsigned)B.ny);
     const int k = B.lo2 + (int)(u / ((unsigned)B.nx * (unsigned)B.ny));
-    const int b = cost_bucket(B.FC.p[aoff(B.FC, i, j, k)]);
+    b = cost_bucket(B.FC.p[aoff(B.FC, i, j, k)]);
+    r = atomicAdd(&cnt[b], 1u);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < COST_BUCKETS; q += blockDim.x)
+    base[q] = (cnt[q] && hist[q] != (unsigned)n) ? atomicAdd(&hist[COST_BUCKETS + q], cnt[q]) : 0u;
+  __syncthreads();
+  if (x < n) {
     // one bucket holding every cell (e.g. FC all zero on a first advance): the given order
-    const unsigned pos = (hist[b] == (unsigned)n) ? (unsigned)x : atomicAdd(&hist[COST_BUCKETS + b], 1u);
+    const unsigned pos = (hist[b] == (unsigned)n) ? (unsigned)x : base[b] + r;
     order[pos] = (int)c;
   }
 }

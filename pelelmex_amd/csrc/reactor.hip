@@ -963,10 +963,11 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
     HIPCHK(hipMemsetAsync(h->d_hist, 0, sizeof(unsigned int) * 2 * prk::COST_BUCKETS, st));
     const int* list = P.cell_list;
     const unsigned int* nlist = P.ncells_dev;
-    const unsigned grid = (unsigned)((ncells + 255) / 256 < 4096 ? (ncells + 255) / 256 : 4096);
-    hipLaunchKernelGGL(prk::cost_hist_kernel, dim3(grid), dim3(256), 0, st, P.one, list, nlist, ncells, h->d_hist);
+    const unsigned hgrid = (unsigned)((ncells + 255) / 256 < 1024 ? (ncells + 255) / 256 : 1024);
+    const unsigned sgrid = (unsigned)((ncells + 255) / 256);  // one cell per thread
+    hipLaunchKernelGGL(prk::cost_hist_kernel, dim3(hgrid), dim3(256), 0, st, P.one, list, nlist, ncells, h->d_hist);
     hipLaunchKernelGGL(prk::cost_scan_kernel, dim3(1), dim3(64), 0, st, h->d_hist);
-    hipLaunchKernelGGL(prk::cost_scatter_kernel, dim3(grid), dim3(256), 0, st, P.one, list, nlist, ncells, h->d_hist,
+    hipLaunchKernelGGL(prk::cost_scatter_kernel, dim3(sgrid), dim3(256), 0, st, P.one, list, nlist, ncells, h->d_hist,
                        h->d_order);
     HIPCHK(hipGetLastError());
     P.cell_list = h->d_order;  // P.ncells_dev (the reject count) is unchanged
