@@ -485,3 +485,33 @@ def test_box_costs(gpu):
     c = r.box_costs([cold, front], "ChemFunctCallAvg")
     assert c[1] > c[0] > 0
     r.close()
+
+
+@pytest.mark.parametrize("lanes", [32, 64])
+def test_too_much_work_status_parity(gpu, lanes):
+    """cvode.max_substeps = 60: stiff cells stop with CVODE's too-much-work code (-1) after their
+    last accepted step; per-cell status, the state written for failed cells (the last accepted
+    step), FC and the failed-cell count are the oracle's, bit for bit."""
+    full = W.ch4_air_flame_box()
+    idx = np.union1d(np.arange(64), np.random.default_rng(9).choice(full["T"].size, 192, replace=False))
+    box = W.subsample(full, idx)
+    ref = _oracle(box, max_order=5, max_steps=60)
+    out = _gpu(box, dict(KEYS, **{"cvode.max_substeps": 60, "hipbdf.lanes_per_cell": lanes}), gpu)
+    failed = ref["status"] != 0
+    assert failed.sum() > 10 and (~failed).sum() > 10        # both kinds present
+    assert np.all(ref["status"][failed] == -1)
+    np.testing.assert_array_equal(out["status"], ref["status"])
+    assert out["nfail"] == int(failed.sum())
+    for k in ("rY", "T", "rH", "FC"):
+        _assert_bits(k, out[k], ref[k])
+
+
+def test_max_order_2_parity(gpu):
+    """cvode.max_order = 2 (the documented default): orders 1-2 only, bit for bit."""
+    full = W.ch4_air_flame_box()
+    box = W.subsample(full, np.random.default_rng(12).choice(full["T"].size, 192, replace=False))
+    ref = _oracle(box, max_order=2)
+    out = _gpu(box, dict(KEYS, **{"cvode.max_order": 2, "hipbdf.lanes_per_cell": 32}), gpu)
+    assert out["nfail"] == 0
+    for k in ("rY", "T", "rH", "FC"):
+        _assert_bits(k, out[k], ref[k])
