@@ -162,10 +162,12 @@ __device__ __forceinline__ int opaque(int v) {
 }
 
 // ---------------------------------------------------------------- group primitives
+// A group of lanes lives inside one wavefront, whose LDS instructions complete in program order:
+// keeping the compiler from moving memory operations across this point is all the ordering the
+// group's LDS traffic needs (no wait for outstanding LDS writes).
 __device__ __forceinline__ void group_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  asm volatile("" ::: "memory");
 }
 
 // One cell per wavefront (G == 64): cross-lane moves stay on the VALU. DPP permutes within rows
