@@ -34,7 +34,8 @@
  *   pr_floor_species_box    PeleLM::floorSpecies per box (Source/PeleLMeX_Utils.cpp:649-690)
  *   pr_heat_release_box     PeleLM::getHeatRelease per box (Source/PeleLMeX_Reactions.cpp:477-506,
  *                    getHGivenT Source/PeleLMeX_K.H:598-615)
- *   pr_surrogate_load, pr_surrogate_features, pr_mlp_*   no reference counterpart: the learned
+ *   pr_surrogate_load, pr_surrogate_features, pr_surrogate_forward, pr_mlp_*   no reference
+ *                    counterpart: the learned
  *                    dt_react map that pr_react_box tries before the BDF (cells failing its FP64
  *                    trust check are integrated by the BDF), and the kernels that train it
  */
@@ -176,6 +177,10 @@ int pr_heat_release_box(pr_handle* h, const int lo[3], const int hi[3], const pr
    accepted cells are re-solved by the BDF and their worst error reported in pr_stats. */
 int pr_surrogate_load(pr_handle* h, int nlayers, const int* dims, const float* params, const float* xnorm,
                       const float* ynorm);
+/* The loaded network on given inputs (no reference counterpart): X device bf16 [ncells x dims[0]]
+   normalised features (as pr_surrogate_features returns them), Y device fp32 [ncells x dims[L]]
+   the raw outputs, the same forward the learned path of pr_react_box runs. Waits for `stream`. */
+int pr_surrogate_forward(pr_handle* h, int64_t ncells, const void* X, float* Y, void* stream);
 /* Network inputs for a batch of states (device, component-major rY[k * ncells + c], T[c],
    dt[c]): raw (fp32 [ncells x (N + 2)], may be NULL) and normalised bf16 X (may be NULL; needs
    a loaded network). */

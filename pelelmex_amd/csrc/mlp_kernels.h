@@ -114,6 +114,95 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmParams P) {
       }
 }
 
+// Forward product of a hidden layer on the inference path: A [M x K] row-major (k fastest, lda =
+// K), Bt = W^T [N x K] row-major (k fastest), K % 32 == 0, N % 128 == 0. Block tile 128 x 128,
+// 256 threads = 4 waves of 64 x 64 (4 x 4 MFMA 16 x 16 x 32 tiles); operand tiles move as 16-byte
+// vectors into double-buffered LDS (pitch 40 bf16), the next K step's loads in flight while the
+// current one multiplies. Per output element the K steps and the MFMA are those of
+// gemm_bf16_kernel, so the two give the same bits. Epilogue: bias, GELU (epi), bf16 store.
+constexpr int FT_M = 128, FT_N = 128;
+
+__global__ void __launch_bounds__(256) gemm_bf16_fwd_kernel(int M, int N, int K, const __bf16* __restrict__ A,
+                                                            const __bf16* __restrict__ Bt,
+                                                            const float* __restrict__ bias, int epi,
+                                                            __bf16* __restrict__ Cb, long ldcb) {
+  __shared__ __align__(16) __bf16 As[2][FT_M * G_PITCH];
+  __shared__ __align__(16) __bf16 Bs[2][FT_N * G_PITCH];
+  const int t = threadIdx.x;
+  const int wave = t / 64, l = t % 64;
+  const int m0 = blockIdx.y * FT_M, n0 = blockIdx.x * FT_N;
+  const int wm = (wave / 2) * 64, wn = (wave % 2) * 64;
+  // staging: 512 16-byte chunks per operand tile (128 rows x 4 chunks of 8 k); thread t moves
+  // chunks t and t + 256 of each operand: row (c / 4), k offset 8 (c % 4)
+  int arow[2], brow[2], kc[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int c = t + 256 * u;
+    const int r = c / 4;
+    kc[u] = 8 * (c % 4);
+    const int gm = m0 + r;
+    arow[u] = gm < M ? gm : M - 1;  // rows past M read a valid row; their results are not stored
+    brow[u] = n0 + r;
+  }
+  bf16x8 ra[2], rb[2];
+  auto gload = [&](int k0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      ra[u] = *reinterpret_cast<const bf16x8*>(A + (long)arow[u] * K + k0 + kc[u]);
+      rb[u] = *reinterpret_cast<const bf16x8*>(Bt + (long)brow[u] * K + k0 + kc[u]);
+    }
+  };
+  auto sstore = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int r = (t + 256 * u) / 4;
+      *reinterpret_cast<bf16x8*>(&As[buf][r * G_PITCH + kc[u]]) = ra[u];
+      *reinterpret_cast<bf16x8*>(&Bs[buf][r * G_PITCH + kc[u]]) = rb[u];
+    }
+  };
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = 0; k0 < K; k0 += GT_K) {
+    const bool more = k0 + GT_K < K;
+    if (more) gload(k0 + GT_K);
+    bf16x8 a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&As[buf][(wm + 16 * i + l % 16) * G_PITCH + 8 * (l / 16)]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&Bs[buf][(wn + 16 * j + l % 16) * G_PITCH + 8 * (l / 16)]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    if (more) sstore(buf ^ 1);  // the other buffer: its last readers passed the previous barrier
+    __syncthreads();
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int gn = n0 + wn + 16 * j + l % 16;
+    const float bj = bias ? bias[gn] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + wm + 16 * i + 4 * (l / 16) + r;
+        if (gm >= M) continue;
+        float v = acc[i][j][r];
+        if (bias) v += bj;
+        if (epi == EPI_GELU) v = gelu_f(v);
+        Cb[(long)gm * ldcb + gn] = (__bf16)v;
+      }
+  }
+}
+
 // dZ = dA * gelu'(Z) (elementwise over rows x cols, row pitches given), as bf16 for the next
 // products, optionally also fp32.
 __global__ void gelu_bwd_kernel(long rows, int cols, const float* dA, long lda, const float* Z, long ldz,

@@ -357,6 +357,7 @@ int parse_keys(const char* text, Keys& K) {
 struct Surrogate {
   std::vector<int> dims;            // layer widths, dims[0] = N + 2, back = N + 1
   std::vector<__bf16*> W;           // [dims[l] x dims[l+1]] bf16
+  std::vector<__bf16*> Wt;          // W^T [dims[l+1] x dims[l]] bf16 (hidden-layer forward kernel)
   std::vector<float*> b;            // [dims[l+1]]
   float *xmu = nullptr, *xsd = nullptr, *ymu = nullptr, *ysd = nullptr, *yeps = nullptr;
   double *Mel = nullptr, *Minv = nullptr;
@@ -651,7 +652,15 @@ static int mlp_forward(pr_handle* h, long ncells, hipStream_t st) {
       G.Cb = out;
       G.ldcb = G.N;
     }
-    HIPCHK(gemm_launch(G, st));
+    if (!last && G.K % prk::GT_K == 0 && G.N % prk::FT_N == 0 && G.K % 8 == 0) {
+      // hidden layer: the tiled forward kernel (same bits as gemm_bf16_kernel)
+      dim3 grid((unsigned)(G.N / prk::FT_N), (unsigned)((G.M + prk::FT_M - 1) / prk::FT_M));
+      hipLaunchKernelGGL(prk::gemm_bf16_fwd_kernel, grid, dim3(256), 0, st, G.M, G.N, G.K, in, S->Wt[l], G.bias,
+                         G.epi, out, (long)G.N);
+      HIPCHK(hipGetLastError());
+    } else {
+      HIPCHK(gemm_launch(G, st));
+    }
     in = out;
   }
   return 0;
@@ -1459,9 +1468,23 @@ int pr_surrogate_load(pr_handle* h, int nlayers, const int* dims, const float* p
       hipLaunchKernelGGL(prk::f32_to_bf16_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, 0, (long)nw, tmp, w);
       ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
     }
+    // the transposed copy for the hidden-layer forward kernel (k fastest in both operands)
+    __bf16* wt = nullptr;
+    if (ok) {
+      std::vector<float> tr(nw);
+      for (int i = 0; i < dims[l]; ++i)
+        for (int o = 0; o < dims[l + 1]; ++o) tr[(size_t)o * dims[l] + i] = params[off + (size_t)i * dims[l + 1] + o];
+      ok = surr_alloc(S, (void**)&wt, nw * sizeof(__bf16)) == 0 &&
+           hipMemcpy(tmp, tr.data(), nw * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
+      if (ok) {
+        hipLaunchKernelGGL(prk::f32_to_bf16_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, 0, (long)nw, tmp, wt);
+        ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+      }
+    }
     (void)hipFree(tmp);
     if (!ok) return cleanup(PR_ERR_HIP, "weight upload failed");
     S->W.push_back(w);
+    S->Wt.push_back(wt);
     S->b.push_back(b);
     off += nw + nb;
   }
@@ -1484,6 +1507,24 @@ int pr_surrogate_load(pr_handle* h, int nlayers, const int* dims, const float* p
     delete h->surr;
   }
   h->surr = S;
+  return 0;
+}
+
+int pr_surrogate_forward(pr_handle* h, int64_t ncells, const void* X, float* Y, void* stream) {
+  if (!h || !X || !Y || ncells < 0) return fail(PR_ERR_ARG, "bad argument");
+  if (!h->surr) return fail(PR_ERR_ARG, "no network loaded");
+  if (ncells == 0) return 0;
+  if (ncells > 0x7fffffffL) return fail(PR_ERR_ARG, "too many rows");
+  HIPCHK(hipSetDevice(h->device));
+  int rc = ensure_surr_buffers(h, (long)ncells);
+  if (rc) return rc;
+  Surrogate* S = h->surr;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(hipMemcpyAsync(S->X, X, sizeof(__bf16) * (size_t)ncells * S->dims[0], hipMemcpyDeviceToDevice, st));
+  if ((rc = mlp_forward(h, (long)ncells, st))) return rc;
+  const int nout = S->dims.back();
+  HIPCHK(hipMemcpyAsync(Y, S->Yout, sizeof(float) * (size_t)ncells * nout, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
   return 0;
 }
 

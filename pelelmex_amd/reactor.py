@@ -24,7 +24,7 @@ _LIB = None
 
 EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_react_batch", "pr_react_batch_enqueue", "pr_collect", "pr_rhs_batch", "pr_jac_batch",
            "pr_reaction_rate_box", "pr_box_costs", "pr_set_temperature_box", "pr_floor_species_box",
-           "pr_heat_release_box", "pr_surrogate_load", "pr_surrogate_features", "pr_mlp_gemm",
+           "pr_heat_release_box", "pr_surrogate_load", "pr_surrogate_features", "pr_surrogate_forward", "pr_mlp_gemm",
            "pr_mlp_gelu_backward", "pr_mlp_column_sum", "pr_mlp_mse", "pr_mlp_adam", "pr_mlp_to_bf16",
            "pr_get_stats", "pr_num_species", "pr_destroy", "pr_last_error"]
 
@@ -97,6 +97,7 @@ def lib():
         vp, i64, fp = C.c_void_p, C.c_int64, C.POINTER(C.c_float)
         L.pr_surrogate_load.argtypes = [vp, C.c_int, C.POINTER(C.c_int), fp, fp, fp]
         L.pr_surrogate_features.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
+        L.pr_surrogate_forward.argtypes = [vp, i64, vp, vp, vp]
         L.pr_mlp_gemm.argtypes = [C.c_int, C.c_int, C.c_int, vp, i64, i64, vp, i64, i64, vp, C.c_int, vp, i64, vp,
                                   i64, vp, i64, C.c_int, vp]
         L.pr_mlp_gelu_backward.argtypes = [i64, C.c_int, vp, i64, vp, i64, vp, i64, vp, i64, vp]
@@ -322,6 +323,19 @@ class Reactor:
                                      xn.ctypes.data_as(fp), yn.ctypes.data_as(fp))
         if rc != 0:
             raise ReactorError(f"pr_surrogate_load failed ({rc}): {lib().pr_last_error().decode()}")
+        self.surrogate_dims = [int(x) for x in d]
+
+    def surrogate_forward(self, X, stream=None):
+        """The loaded network on normalised bf16 features X [nc, dims[0]] (device); fp32 [nc, dims[-1]]."""
+        import torch
+        nc = X.shape[0]
+        nout = self.surrogate_dims[-1]
+        Y = torch.empty((nc, nout), dtype=torch.float32, device=X.device)
+        st = stream if stream is not None else torch.cuda.current_stream()
+        rc = lib().pr_surrogate_forward(self._h, nc, X.data_ptr(), Y.data_ptr(), C.c_void_p(st.cuda_stream))
+        if rc != 0:
+            raise ReactorError(f"pr_surrogate_forward failed ({rc}): {lib().pr_last_error().decode()}")
+        return Y
 
     def surrogate_features(self, rY, T, dt, normalised=False, stream=None):
         """Network inputs for a batch (device tensors rY [N, nc], T [nc], dt [nc]): raw fp32

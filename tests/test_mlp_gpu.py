@@ -111,3 +111,27 @@ def test_adam_against_closed_form(gpu):
     # the bf16 copies follow the fp32 parameters
     for w, wb in zip(mlp.W, mlp.Wb):
         np.testing.assert_array_equal(wb.float().cpu().numpy(), mo.bf16_round(w.cpu().numpy()).astype(np.float32))
+
+
+def test_hidden_layer_forward_kernel_bits(gpu):
+    """pr_surrogate_forward (the learned path's forward: the tiled 128 x 128 kernel with
+    transposed weights for the hidden layers) gives the bits of the same network run layer by
+    layer through pr_mlp_gemm, on a row count that is not a multiple of the tile."""
+    import torch
+    from pelelmex_amd import surrogate as sg
+    from pelelmex_amd.reactor import Reactor
+    model = sg.load(sg.WEIGHTS)
+    dims = [int(d) for d in model["dims"]]
+    mlp = sg.MLP(dims, device=gpu)
+    mlp.set_flat_params(torch.as_tensor(model["params"], dtype=torch.float32))
+    r = Reactor.create("ReactorCvode", "drm19", {"ode.rtol": 1e-10, "ode.atol": 1e-10})
+    r.init(2, 1)
+    sg.attach(r, model)
+    g = torch.Generator(device="cpu").manual_seed(3)
+    X = (torch.randn((1000, dims[0]), generator=g) * 1.5).to(torch.bfloat16).to(gpu)
+    Y1 = r.surrogate_forward(X)
+    Y2 = mlp.forward(X)
+    torch.cuda.synchronize()
+    assert torch.isfinite(Y1).all()
+    assert torch.equal(Y1, Y2)
+    r.close()
