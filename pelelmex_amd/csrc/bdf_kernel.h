@@ -487,10 +487,15 @@ struct Cell {
   static constexpr int n = N + 1;
   static constexpr int RPL = (n + G - 1) / G;
   static constexpr int RXL = (R + G - 1) / G;
-  // With one cell per wavefront the saved Jacobian and its records sit in LDS after the group's
-  // scratch (few groups per block); narrower groups keep them in a global scratch slot.
-  static constexpr bool js_lds = (G == 64);
-  static constexpr int lds_doubles = Smem<N, R>::scratch_doubles + n * n + (js_lds ? Smem<N, R>::slot_doubles : 0);
+  // Above 32 components (one row per lane of a 64-lane group) the LU factors stay in registers
+  // for the whole integration and the group's LDS holds no n x n area: the Jacobian is assembled
+  // in its global slot.
+  static constexpr bool lu_resident = (G == 64 && n > 32);
+  // Otherwise, with one cell per wavefront the saved Jacobian and its records sit in LDS after
+  // the group's scratch (few groups per block); narrower groups keep them in a global slot.
+  static constexpr bool js_lds = (G == 64 && !lu_resident);
+  static constexpr int lds_doubles =
+      Smem<N, R>::scratch_doubles + (lu_resident ? 0 : n * n) + (js_lds ? Smem<N, R>::slot_doubles : 0);
   static constexpr int gslot_doubles = js_lds ? 0 : Smem<N, R>::slot_doubles;
 };
 
@@ -660,13 +665,15 @@ __device__ __forceinline__ void rate_rhoY_eval(const Smem<N, R>& S, int lane, co
 // ---------------------------------------------------------------- analytic Jacobian
 // Exact Jacobian of the right-hand side with T = y_T, written into S.Js (row-major). Rows are
 // assembled in LDS (the saved Jacobian itself at G = 64, else the LU area, which is dead here:
-// a factorization always follows) and then copied to the global slot.
+// a factorization always follows) and then copied to the global slot; with register-resident
+// factors there is no LU area and they are assembled in the global slot itself (each lane
+// updates its own row; a wavefront's global accesses to one address complete in order).
 template <int N, int R, int G>
 __device__ __forceinline__ void jac_eval(const Smem<N, R>& S, int lane, const double (&y)[Cell<N, R, G>::RPL],
                                          const double (&src)[Cell<N, R, G>::RPL], double rhohsrc) {
   constexpr int RPL = Cell<N, R, G>::RPL;
   constexpr int n = N + 1;
-  double* const A = Cell<N, R, G>::js_lds ? S.Js : S.LU;  // assembly rows (LDS)
+  double* const A = (Cell<N, R, G>::js_lds || Cell<N, R, G>::lu_resident) ? S.Js : S.LU;  // assembly rows
   const double T = gbcast<G>(y[N / G], N % G);
   const double lnT = pr_log(T), invT = 1.0 / T, lnRTP = pr_log((PR_RU * T) / PR_PATM);
   double cc[RPL], cpk[RPL], dcpk[RPL];
@@ -771,7 +778,7 @@ __device__ __forceinline__ void jac_eval(const Smem<N, R>& S, int lane, const do
     }
   }
   group_sync();
-  if constexpr (!Cell<N, R, G>::js_lds) {
+  if constexpr (!Cell<N, R, G>::js_lds && !Cell<N, R, G>::lu_resident) {
     for (int e = lane; e < n * n; e += G) S.Js[e] = A[e];
     group_sync();
   }

@@ -21,6 +21,7 @@
 #include "eos_kernel.h"
 #include "mech/drm19.h"
 #include "mech/lidryer.h"
+#include "mech/syn53.h"
 
 namespace {
 
@@ -37,7 +38,7 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(PR_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-const pr_mech_t* const kMechs[] = {&pr_lidryer, &pr_drm19};
+const pr_mech_t* const kMechs[] = {&pr_lidryer, &pr_drm19, &pr_syn53};
 
 const pr_mech_t* find_mech(const char* name) {
   if (!name) return nullptr;
@@ -152,18 +153,22 @@ struct Variant {
 // CU): some registers spill, yet it is 18% faster than 2 waves with none (the integrator is
 // latency bound). At 64 lanes the group's LDS does not fit 3 waves; at 16 lanes 4 cells share
 // a wave and registers already spill at 2. LiDryer at 16 lanes fits 3 waves without spills
-// (21% faster than 2 on a 32^3 box); at 8 lanes 4 waves spill and are slower.
+// (21% faster than 2 on a 32^3 box); at 8 lanes 4 waves spill and are slower. A 54-component
+// state (syn53) keeps its LU factors in registers (108 VGPRs): at 2 waves 128 VGPRs spill to
+// scratch memory, at 1 wave none (the AGPRs hold them).
 const Variant kVariants[] = {
     PR_VARIANT("LiDryer", PR_LIDRYER_NSP, PR_LIDRYER_NR, 16, 3),
     PR_VARIANT("LiDryer", PR_LIDRYER_NSP, PR_LIDRYER_NR, 8, 2),
     PR_VARIANT("drm19", PR_DRM19_NSP, PR_DRM19_NR, 32, 3),
     PR_VARIANT("drm19", PR_DRM19_NSP, PR_DRM19_NR, 16, 2),
     PR_VARIANT("drm19", PR_DRM19_NSP, PR_DRM19_NR, 64, 2),
+    PR_VARIANT("syn53", PR_SYN53_NSP, PR_SYN53_NR, 64, 2),
+    PR_VARIANT("syn53", PR_SYN53_NSP, PR_SYN53_NR, 64, 1),
 };
 
-const Variant* find_variant(const char* mech, int G) {
+const Variant* find_variant(const char* mech, int G, int W = 0) {
   for (const Variant& v : kVariants)
-    if (std::strcmp(v.mech, mech) == 0 && (G == 0 || v.G == G)) return &v;
+    if (std::strcmp(v.mech, mech) == 0 && (G == 0 || v.G == G) && (W == 0 || v.waves == W)) return &v;
   return nullptr;
 }
 
@@ -247,7 +252,7 @@ struct Keys {
   int max_order = 2, max_substeps = 10000;
   std::string solve_type = "GMRES";
   int clean_init_massfrac = 0, analytical_jacobian = 1, verbose = 0;
-  int lanes_per_cell = 0, block = 0, autotune = 0;
+  int lanes_per_cell = 0, block = 0, autotune = 0, waves = 0;
   int cost_order = 1;  // hand cells out by decreasing input FC (PeleLMeX passes the previous advance's)
   // learned map in front of the BDF (used once weights are loaded with pr_surrogate_load)
   int surr_enable = 1;
@@ -314,6 +319,9 @@ int parse_keys(const char* text, Keys& K) {
     } else if (k == "hipbdf.lanes_per_cell") {
       if (!num(d)) return fail(PR_ERR_KEY, "bad hipbdf.lanes_per_cell");
       K.lanes_per_cell = (int)d;
+    } else if (k == "hipbdf.waves_per_simd") {
+      if (!num(d) || d < 0 || d > 4 || d != (int)d) return fail(PR_ERR_KEY, "hipbdf.waves_per_simd must be 0 to 4");
+      K.waves = (int)d;
     } else if (k == "hipbdf.cost_order") {
       if (!num(d) || (d != 0 && d != 1)) return fail(PR_ERR_KEY, "hipbdf.cost_order must be 0 or 1");
       K.cost_order = (int)d;
@@ -422,8 +430,10 @@ int pr_create(const char* integrator, const char* mechanism, int reactor_type, i
     return fail(PR_ERR_KEY, "ode.clean_init_massfrac must be 0 or 1");
   int G = K.lanes_per_cell;
   if (G == 0) G = (m->nsp + 1 <= 16) ? 16 : (m->nsp + 1 <= 32 ? 32 : 64);
-  const Variant* v = find_variant(m->name, G);
-  if (!v) return fail(PR_ERR_UNSUPPORTED, "no kernel variant for this mechanism and hipbdf.lanes_per_cell");
+  const Variant* v = find_variant(m->name, G, K.waves);
+  if (!v)
+    return fail(PR_ERR_UNSUPPORTED,
+                "no kernel variant for this mechanism, hipbdf.lanes_per_cell and hipbdf.waves_per_simd");
   auto* h = new pr_handle();
   h->mech = m;
   h->var = v;

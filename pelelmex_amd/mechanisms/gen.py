@@ -29,7 +29,7 @@ import sys
 
 ATOMIC_WEIGHTS = {"H": 1.008, "C": 12.011, "O": 15.999, "N": 14.007, "Ar": 39.95}
 R_CAL = 8.314462618 / 4.184  # cal / (mol K)
-MECHANISMS = ["lidryer", "drm19"]
+MECHANISMS = ["lidryer", "drm19", "syn53"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 OUT_DIR = os.path.join(os.path.dirname(_HERE), "csrc", "mech")

@@ -20,6 +20,13 @@ Recipes (documented choices where the configuration text leaves them open):
   dissociation radicals H, O, OH each Y ~ 10^U[-6, -3], T ~ U[1800, 2300] K; renormalised.
   External sources rhoY_src_k = 0.05 |rhoY_k| / dt * xi_k and rhoH_src = 0.05 |rhoH| / dt * xi_h
   with xi ~ N(0, 1); P = 1 atm; dt = 5e-6 s.
+* syn53 at the n-dodecane size (synthetic 53-species mechanism, 128^3 in eight 64^3 chunks,
+  seed 3003 + chunk): mixture fraction Z ~ Beta(0.5, 2) between a CH4 fuel stream and air
+  (O2:N2 = 1 : 3.76), phi = (Z / (1 - Z)) / (Z_st / (1 - Z_st)) with Z_st = 0.0552; burn
+  fraction c ~ U[0, 1] toward the CH4 products at min(phi, 2); radicals H, O, OH, HO2, CH3,
+  HCO each Y ~ 10^U[-8, -3] and every isomer X01..X32 Y ~ 10^U[-8, -4]; renormalised;
+  T ~ U[700, 2400] K; P = 60 atm; no sources; dt = 2e-6 s; 10 % of cells masked off (mask -1)
+  at random.
 """
 from __future__ import annotations
 
@@ -127,9 +134,33 @@ def ch4_air_flame_box(nx=64, ny=64, nz=64, seed=2002, dt=5e-6, src_frac=0.05):
     return _finish(m, Y, T, PATM, dt, src_frac, rng, (nx, ny, nz))
 
 
+def syn53_chunk(chunk=0, nx=64, ny=64, nz=64, seed=3003, dt=2e-6, masked=0.10):
+    """One 64^3 chunk (262144 cells) of the 128^3 syn53 box at 60 atm."""
+    m = load("syn53")
+    rng = _rng(seed + int(chunk))
+    n = nx * ny * nz
+    Z = rng.beta(0.5, 2.0, n)
+    zst = 0.0552
+    phi = (Z / np.maximum(1.0 - Z, 1e-12)) / (zst / (1.0 - zst))
+    c = rng.uniform(0.0, 1.0, n)
+    T = rng.uniform(700.0, 2400.0, n)
+    Y_air = m.X_to_Y(np.array([[1.0 if s == "O2" else (3.76 if s == "N2" else 0.0) for s in m.species]]))[0]
+    Y_u = (1.0 - Z)[:, None] * Y_air[None, :]
+    Y_u[:, m.index["CH4"]] += Z
+    Y_b = m.X_to_Y(_ch4_products(m, np.minimum(phi, 2.0)))
+    Y = (1 - c)[:, None] * Y_u + c[:, None] * Y_b
+    for s in ("H", "O", "OH", "HO2", "CH3", "HCO"):
+        Y[:, m.index[s]] += 10.0 ** rng.uniform(-8.0, -3.0, n)
+    for s in m.species[21:]:
+        Y[:, m.index[s]] += 10.0 ** rng.uniform(-8.0, -4.0, n)
+    mask = np.where(rng.uniform(0.0, 1.0, n) < masked, -1, 1).astype(np.int32)
+    return _finish(m, Y, T, 60.0 * PATM, dt, 0.0, rng, (nx, ny, nz), mask=mask)
+
+
 CONFIGS = {
     "h2_air_32": h2_air_box,
     "ch4_air_flame_64": ch4_air_flame_box,
+    "syn53_60atm_64": syn53_chunk,
 }
 
 

@@ -130,7 +130,7 @@ def _random_state(mech, rng, T=1500.0):
     return m, Y, rho
 
 
-@pytest.mark.parametrize("name", ["LiDryer", "drm19"])
+@pytest.mark.parametrize("name", ["LiDryer", "drm19", "syn53"])
 def test_mass_and_element_conservation(name):
     rng = np.random.default_rng(3)
     om = po.Mech(name)
@@ -142,6 +142,36 @@ def test_mass_and_element_conservation(name):
         assert abs(np.sum(w * m.W)) <= 1e-13 * scale
         el = m.elem_matrix.T @ w
         assert np.all(np.abs(el) <= 1e-12 * np.abs(w).max())
+
+
+def test_syn53_isomer_equilibrium_constants():
+    """syn53's isomers carry their parent's NASA-7 fit with a6 + dH/R and a7 + dS/R, so every
+    isomerization X + B <=> P + B (and X (+M) <=> P (+M), X + M <=> P + M) has
+    k_f / k_r = exp(g_X/RT - g_P/RT) = exp(dH/(R T) - dS/R) exactly: the oracle's forward and
+    reverse rates must show that ratio at any state."""
+    from pelelmex_amd.mechanisms import syn53
+    om = po.Mech("syn53")
+    m = load("syn53")
+    rng = np.random.default_rng(7)
+    checked = 0
+    for T in (800.0, 1400.0, 2300.0):
+        Y = rng.random(m.nsp) + 1e-3
+        Y /= Y.sum()
+        C = m.density(60 * PATM, T, Y) * Y / m.W
+        f, b = om.rates_fr(C, T)
+        for j, r in enumerate(m.reactions):
+            if j < 84 or len(r.reac) > 2 or r.reac[0] not in syn53.PARENT:
+                continue
+            x = r.reac[0]
+            p = syn53.PARENT[x]
+            if r.prod[0] != p or (len(r.reac) == 2 and r.reac[1] in syn53.PARENT):
+                continue
+            i = int(x[1:]) - 1
+            dh, ds = 2500.0 + 250.0 * (i % 8), 0.5 - 0.25 * (i % 3)
+            ratio = (f[j] / C[m.index[x]]) / (b[j] / C[m.index[p]])
+            assert abs(ratio / np.exp(dh / T - ds) - 1) < 1e-11, (r.eq, T)
+            checked += 1
+    assert checked == 3 * 160
 
 
 def test_falloff_limits():
@@ -176,10 +206,12 @@ def test_falloff_limits():
 def _cells():
     b = W.ch4_air_flame_box(nx=64, ny=2, nz=2, seed=31)
     h = W.h2_air_box(nx=8, ny=2, nz=2, seed=32)
-    return [(b, c) for c in (0, 25, 40, 60)] + [(h, c) for c in (0, 9)]
+    s = W.syn53_chunk(0, nx=8, ny=2, nz=2)
+    act = [c for c in range(32) if s["mask"][c] == 1]
+    return [(b, c) for c in (0, 25, 40, 60)] + [(h, c) for c in (0, 9)] + [(s, c) for c in act[:2]]
 
 
-@pytest.mark.parametrize("bc", range(6))
+@pytest.mark.parametrize("bc", range(8))
 def test_jacobian_vs_finite_differences(bc):
     box, c = _cells()[bc]
     m = po.Mech(box["mech"])
