@@ -65,7 +65,20 @@ def profiled_traffic(workload_key):
     return None, None
 
 
-def cpu_baseline(box, sample, seed=7):
+def copy_probe(mb, dev):
+    """One plain device copy of `mb` MiB (its byte counters calibrate a profile's FETCH / WRITE sizes)."""
+    if not mb:
+        return
+    import torch
+    a = torch.empty(mb * (1 << 20) // 8, dtype=torch.float64, device=dev).fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def cpu_baseline(box, sample, seed=7, what="the same 64^3 box"):
     """CPU oracle on `sample` random cells of the box; returns the cpu_baseline object."""
     from oracle import pyoracle as po
     from pelelmex_amd import workloads as W
@@ -79,7 +92,7 @@ def cpu_baseline(box, sample, seed=7):
                   max_order=4, nthreads=nthreads)
     dt = time.perf_counter() - t0
     return {"value": sample / dt, "unit": "cells/s", "cores": nthreads, "kind": "port",
-            "sample": f"{sample} cells (seeded random order) of the same 64^3 box, CPU oracle BDF (C, OpenMP "
+            "sample": f"{sample} cells (seeded random order) of {what}, CPU oracle BDF (C, OpenMP "
                       f"dynamic over cells), {dt:.1f} s, failures {int(np.sum(out['status'] != 0))}"}
 
 
@@ -97,11 +110,20 @@ def main():
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device copy of this size (calibrates byte counters in a profile)")
+    ap.add_argument("--waves", type=int, default=0,
+                    help="syn53_128: hipbdf.waves_per_simd (with --block; 0: the autotuner chooses)")
+    ap.add_argument("--block", type=int, default=0, help="syn53_128: hipbdf.block (with --waves)")
+    ap.add_argument("--workload", default="drm19_64", choices=["drm19_64", "syn53_128"],
+                    help="syn53_128: plain GPU BDF on the synthetic 53-species mechanism, 128^3 cells at 60 atm "
+                         "in eight 262144-cell launches (one step = all eight)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X; this build has no multi-GPU path (each GPU would run its own "
               "boxes independently).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "syn53_128":
+        bench_syn53(args)
+        return
 
     import torch
     from pelelmex_amd import roofline, surrogate, workloads
@@ -140,13 +162,7 @@ def main():
         det = bool(torch.equal(ref_T, t["T"])) if ref_T is not None else None
         return ms, kstats, det, {k: t[k].clone() for k in ("rY", "T", "FC")}
 
-    if args.copy_probe_mb:
-        a = torch.empty(args.copy_probe_mb * (1 << 20) // 8, dtype=torch.float64, device=dev).fill_(1.0)
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
+    copy_probe(args.copy_probe_mb, dev)
 
     r = Reactor.create("ReactorCvode", "drm19", keys)
     r.init(2, 1)
@@ -233,6 +249,101 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(box, min(args.cpu_sample, ncells))
+    print(json.dumps(line), flush=True)
+
+
+def bench_syn53(args):
+    """Plain GPU BDF at the n-dodecane size: syn53 (synthetic 53 species, 268 reactions), a 128^3 box
+    at 60 atm with 10 % of cells masked, dt_react 2e-6 s, as eight 64^3 chunks of 262144 cells (one
+    react launch each). One step = the eight launches over the whole box, each chunk restored from
+    a pristine device copy first. Lanes per cell, waves per SIMD and block size are chosen by the
+    reactor's autotuner on the first (warm-up) call, or fixed by --waves / --block."""
+    import torch
+    from pelelmex_amd import roofline, workloads
+    from pelelmex_amd.reactor import Reactor, box_tensors
+    dev = torch.device("cuda:0")
+    nchunk = 8
+    chunks, pristine, host0 = [], [], None
+    for c in range(nchunk):
+        b = workloads.syn53_chunk(c)
+        if c == 0:
+            host0 = b
+        t = box_tensors(b, dev)
+        chunks.append((b, t))
+        pristine.append({k: t[k].clone() for k in ("rY", "T", "rH")})
+    ncells = sum(b["T"].size for b, _ in chunks)
+    active = sum(int((b["mask"] != -1).sum()) for b, _ in chunks)
+    copy_probe(args.copy_probe_mb, dev)
+    keys = {"ode.rtol": 1e-10, "ode.atol": 1e-10, "cvode.max_order": args.max_order, "cvode.max_substeps": 10000,
+            "cvode.solve_type": "denseAJ_direct"}
+    if args.waves:
+        keys["hipbdf.waves_per_simd"] = args.waves
+        if args.block:
+            keys["hipbdf.block"] = args.block
+    else:
+        keys["hipbdf.autotune"] = 1
+    r = Reactor.create("ReactorCvode", "syn53", keys)
+    r.init(2, 1)
+
+    def launch(c):
+        b, t = chunks[c]
+        for k in pristine[c]:
+            t[k].copy_(pristine[c][k])
+        nfail, _, _ = r.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], t["mask"], b["dt"], 0.0)
+        if nfail:
+            raise RuntimeError(f"{nfail} cells failed to integrate")
+        return r.stats()
+
+    print(f"syn53: {ncells} cells generated and resident; warm-up (autotune) on chunk 0", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        launch(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(args.steps):
+        stats = [launch(c) for c in range(nchunk)]
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    r.close()
+    tot = {k: sum(s[k] for s in stats) for k in ("nfe", "nje", "nsetups", "nni", "nst", "netf", "cells")}
+    kernel_ms = sum(s["kernel_ms"] for s in stats)
+    F = roofline.flops("syn53", tot)
+    achieved = F / (kernel_ms / 1e3) / 1e12
+    counts = roofline.static_counts("syn53")
+    st = stats[-1]
+    workload_key = f"syn53 128^3 lanes {st['lanes_per_cell']} waves {st['waves_per_simd']} max_order {args.max_order}"
+    traffic, traffic_src = profiled_traffic(workload_key)
+    line = {
+        "metric": "chem cells/s per dt_react: plain GPU BDF, synthetic 53-species mechanism (n-dodecane size)",
+        "value": ncells / (ms / 1e3),
+        "unit": "cells/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "syn53 (synthetic: 53 species, 268 reactions) 128^3 = 2097152 cells in 8 launches of "
+                               "262144, 60 atm, dt_react 2e-6 s, 10% masked, mixture fraction ~ Beta(0.5, 2), "
+                               "T ~ U[700, 2400] K",
+                   "mechanism": "syn53", "cells": ncells, "active_cells": active, "dt_react": host0["dt"],
+                   "rtol": 1e-10, "atol": 1e-10, "max_order": args.max_order,
+                   "lanes_per_cell": st["lanes_per_cell"], "waves_per_simd": st["waves_per_simd"],
+                   "block_threads": st["block_threads"], "seed": "3003 + chunk",
+                   "path": "plain GPU BDF (" + ("fixed variant" if args.waves else "autotuned") + ")"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": roofline.MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / roofline.MI355X_FP64_TFLOPS, "traffic": traffic,
+                     "traffic_source": (f"profiles/{traffic_src} (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a "
+                                        "1 GiB device copy)") if traffic else "no profile of these kernel sources",
+                     "kernel": "react_kernel<53,268,%d,%d> (%d-thread blocks)" % (
+                         st["lanes_per_cell"], st["waves_per_simd"], st["block_threads"]),
+                     "launches_per_step": nchunk, "kernel_ms_per_step": kernel_ms,
+                     "kernel_ms_avg_launch": kernel_ms / nchunk, "flops_per_step": F,
+                     "traffic_per": "launch", "hbm_algorithmic_bytes_per_launch": counts["bytes_cell"] * ncells // nchunk,
+                     "hbm_algorithmic_GBs": counts["bytes_cell"] * ncells / (kernel_ms / 1e3) / 1e9},
+        "integrator": {"steps_per_cell": tot["nst"] / tot["cells"], "rhs_per_cell": tot["nfe"] / tot["cells"],
+                       "jac_per_cell": tot["nje"] / tot["cells"], "lu_per_cell": tot["nsetups"] / tot["cells"],
+                       "newton_per_cell": tot["nni"] / tot["cells"], "err_fail_per_cell": tot["netf"] / tot["cells"]},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(host0, min(args.cpu_sample, 65536),
+                                            what="the first 64^3 chunk (masked cells count as cells)")
     print(json.dumps(line), flush=True)
 
 
