@@ -56,6 +56,7 @@ struct BoxDesc {
   int has_mask;
   int* status;  // per cell (linear box index), nullable
   double dt;
+  const double* dtv;  // per-cell dt_react (linear box index), nullable: dt for every cell
 };
 
 struct KParams {

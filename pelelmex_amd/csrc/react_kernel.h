@@ -414,9 +414,8 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
       cl = __shfl(cl, 0, G);
       if ((long)cl >= nwork) break;
       int b = 0;
-      if (P.cell_list) {
-        cl = (unsigned int)P.cell_list[cl];
-      } else if (P.nbox > 1) {  // the box holding queue entry cl
+      if (P.cell_list) cl = (unsigned int)P.cell_list[cl];  // a listed entry: box-local (one box) or global
+      if (P.nbox > 1) {  // the box holding entry cl
         int lo = 0, hi = P.nbox;
         while (hi - lo > 1) {
           const int mid = (lo + hi) >> 1;
@@ -431,7 +430,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
       cell_ijk(B, (int)cl, i, j, k);
       c.cell = (int)cl;
       c.box = b;
-      c.tout = B.dt;
+      c.tout = B.dtv ? B.dtv[cl] : B.dt;
       if (B.has_mask) {
         long mo = (long)(i - B.mask.bx) + (long)(j - B.mask.by) * B.mask.js + (long)(k - B.mask.bz) * B.mask.ks;
         if (B.mask.p[mo] == -1) {
@@ -1086,6 +1085,10 @@ __global__ void __launch_bounds__(256) box_cost_kernel(const CostBox* boxes, int
                       : (method == COST_FC_MAX) ? smax[0]
                                                 : ssum[0];
   }
+}
+
+__global__ void zero_u32(unsigned int* p, int n) {
+  for (int t = threadIdx.x; t < n; t += blockDim.x) p[t] = 0u;
 }
 
 __global__ void zero_counters(unsigned int* queue, unsigned long long* stats, unsigned int* nfail) {

@@ -380,10 +380,13 @@ struct Surrogate {
   unsigned int aud_cap = 0;
   double *aud_in = nullptr, *aud_out = nullptr, *aud_box = nullptr, *aud_err = nullptr;
   int* aud_cell = nullptr;
+  double* aud_dt = nullptr;         // [aud_cap] dt_react of each audited cell
   std::vector<void*> owned;
 };
 
 struct pr_handle {
+  int batch_surr = 0;            // the last pr_react_batch_enqueue ran the learned map (pr_collect reads its counters)
+  unsigned int batch_acap = 0;   // and audited up to this many cells
   const pr_mech_t* mech = nullptr;
   const Variant* var = nullptr;
   Keys keys;
@@ -594,7 +597,7 @@ static int ensure_surr_buffers(pr_handle* h, long ncells) {
   if (ncells <= S->cap) return 0;
   const int N = h->mech->nsp;
   for (void* q : {(void*)S->X, (void*)S->H0, (void*)S->H1, (void*)S->Yout, (void*)S->Tstar, (void*)S->scale, (void*)S->rej, (void*)S->aud_in,
-                  (void*)S->aud_out, (void*)S->aud_box, (void*)S->aud_err, (void*)S->aud_cell})
+                  (void*)S->aud_out, (void*)S->aud_box, (void*)S->aud_err, (void*)S->aud_cell, (void*)S->aud_dt})
     if (q) {
       (void)hipFree(q);
       for (auto it = S->owned.begin(); it != S->owned.end(); ++it)
@@ -622,7 +625,8 @@ static int ensure_surr_buffers(pr_handle* h, long ncells) {
       (rc = surr_alloc(S, (void**)&S->aud_out, sizeof(double) * ac * (N + 1))) ||
       (rc = surr_alloc(S, (void**)&S->aud_box, sizeof(double) * ac * (2 * N + 4))) ||
       (rc = surr_alloc(S, (void**)&S->aud_err, sizeof(double) * 2 * ac)) ||
-      (rc = surr_alloc(S, (void**)&S->aud_cell, sizeof(int) * ac)))
+      (rc = surr_alloc(S, (void**)&S->aud_cell, sizeof(int) * ac)) ||
+      (rc = surr_alloc(S, (void**)&S->aud_dt, sizeof(double) * ac)))
     return rc;
   S->cap = ncells;
   return 0;
@@ -635,14 +639,15 @@ static hipError_t gemm_launch(const prk::GemmParams& G, hipStream_t st) {
   return hipGetLastError();
 }
 
-// Network forward over ncells feature rows S->X: hidden layers GELU into bf16, last layer fp32.
-static int mlp_forward(pr_handle* h, long ncells, hipStream_t st) {
+// Network forward over feature rows [r0, r0 + m) of S->X: hidden layers GELU into bf16, last
+// layer fp32 into the same rows of S->Yout.
+static int mlp_forward_rows(pr_handle* h, long r0, long m, hipStream_t st) {
   Surrogate* S = h->surr;
   const int nl = (int)S->dims.size() - 1;
-  const __bf16* in = S->X;
+  const __bf16* in = S->X + r0 * S->dims[0];
   for (int l = 0; l < nl; ++l) {
     prk::GemmParams G{};
-    G.M = (int)ncells;
+    G.M = (int)m;
     G.N = S->dims[l + 1];
     G.K = S->dims[l];
     G.A = in;
@@ -654,9 +659,9 @@ static int mlp_forward(pr_handle* h, long ncells, hipStream_t st) {
     G.bias = S->b[l];
     const bool last = (l == nl - 1);
     G.epi = last ? prk::EPI_NONE : prk::EPI_GELU;
-    __bf16* out = (l % 2 == 0) ? S->H0 : S->H1;
+    __bf16* out = ((l % 2 == 0) ? S->H0 : S->H1) + r0 * G.N;
     if (last) {
-      G.C = S->Yout;
+      G.C = S->Yout + r0 * G.N;
       G.ldc = G.N;
     } else {
       G.Cb = out;
@@ -672,6 +677,17 @@ static int mlp_forward(pr_handle* h, long ncells, hipStream_t st) {
       HIPCHK(gemm_launch(G, st));
     }
     in = out;
+  }
+  return 0;
+}
+
+// Network forward over ncells feature rows S->X into S->Yout.
+// Rows go in chunks whose row tiles fit the grid's y dimension (at most 65535 blocks).
+static int mlp_forward(pr_handle* h, long ncells, hipStream_t st) {
+  constexpr long kRows = 32767L * 128;  // a multiple of both kernels' row tiles, 65535 * 64 at most
+  for (long r0 = 0; r0 < ncells; r0 += kRows) {
+    const long m = ncells - r0 < kRows ? ncells - r0 : kRows;
+    if (int rc = mlp_forward_rows(h, r0, m, st)) return rc;
   }
   return 0;
 }
@@ -771,6 +787,206 @@ static int autotune(pr_handle* h, const prk::KParams& Pbase, const prk::BoxDesc&
   return 0;
 }
 
+// Blocks for a grid-stride launch of the surrogate's per-cell kernels over ncells cells.
+static int surr_blocks(pr_handle* h, long ncells, int* block, size_t* shmem, int* gblocks) {
+  const int G = h->var->G;
+  *block = pick_block(h);
+  *shmem = (size_t)h->blob_bytes + (size_t)(*block / G) * h->var->scratch;
+  int dev_cus = 0;
+  HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, h->device));
+  const long want = (ncells + (*block / G) - 1) / (*block / G);
+  *gblocks = (int)(want < 8L * dev_cus ? want : 8L * dev_cus);
+  if (*gblocks < 1) *gblocks = 1;
+  return 0;
+}
+
+// The learned map's first half for one box: features and rate scales of its cells, written to
+// rows [base, base + cells) of the surrogate buffers.
+static int surr_prep(pr_handle* h, const prk::BoxDesc& d, long base, hipStream_t st) {
+  Surrogate* S = h->surr;
+  const int N = h->mech->nsp;
+  int block = 0, gblocks = 0;
+  size_t shmem = 0;
+  if (int rc = surr_blocks(h, d.ncells, &block, &shmem, &gblocks)) return rc;
+  prk::PrepParams F{};
+  F.lo0 = d.lo0;
+  F.lo1 = d.lo1;
+  F.lo2 = d.lo2;
+  F.nx = d.nx;
+  F.ny = d.ny;
+  F.ncells = d.ncells;
+  F.rY = d.rY;
+  F.rYsrc = d.rYsrc;
+  F.T = d.T;
+  F.rH = d.rH;
+  F.rHsrc = d.rHsrc;
+  F.mask = d.mask;
+  F.has_mask = d.has_mask;
+  F.clean = h->keys.clean_init_massfrac;
+  F.dt = d.dt;
+  F.xmu = S->xmu;
+  F.xsd = S->xsd;
+  F.X = S->X + base * (N + 2);
+  F.Tstar = S->Tstar + base;
+  F.scale = S->scale + base * (N + 1);
+  F.blob = h->d_blob;
+  F.blob_bytes = h->blob_bytes;
+  F.neff = h->mech->neff;
+  F.ninc = h->mech->ninc;
+  HIPCHK(h->var->prep(F, gblocks, block, shmem, st));
+  return 0;
+}
+
+// The second half, after the network has run on rows [base, base + cells): the FP64 trust check
+// writes accepted cells in place and lists the others as base + box-local index.
+static int surr_trust(pr_handle* h, const prk::BoxDesc& d, long base, hipStream_t st) {
+  Surrogate* S = h->surr;
+  const int N = h->mech->nsp;
+  int block = 0, gblocks = 0;
+  size_t shmem = 0;
+  if (int rc = surr_blocks(h, d.ncells, &block, &shmem, &gblocks)) return rc;
+  prk::TrustParams Q{};
+  Q.lo0 = d.lo0;
+  Q.lo1 = d.lo1;
+  Q.lo2 = d.lo2;
+  Q.nx = d.nx;
+  Q.ny = d.ny;
+  Q.ncells = d.ncells;
+  Q.rY = d.rY;
+  Q.rYsrc = d.rYsrc;
+  Q.T = d.T;
+  Q.rH = d.rH;
+  Q.rHsrc = d.rHsrc;
+  Q.FC = d.FC;
+  Q.mask = d.mask;
+  Q.has_mask = d.has_mask;
+  Q.clean = h->keys.clean_init_massfrac;
+  Q.status = d.status;
+  Q.dt = d.dt;
+  Q.Yout = S->Yout + base * (N + 1);
+  Q.ymu = S->ymu;
+  Q.ysd = S->ysd;
+  Q.yeps = S->yeps;
+  Q.Mel = S->Mel;
+  Q.Minv = S->Minv;
+  Q.E = S->E;
+  Q.trust_rtol = h->keys.surr_trust_rtol;
+  Q.trust_atol = h->keys.surr_trust_atol;
+  Q.elem_tol = h->keys.surr_elem_tol;
+  Q.blob = h->d_blob;
+  Q.blob_bytes = h->blob_bytes;
+  Q.neff = h->mech->neff;
+  Q.ninc = h->mech->ninc;
+  Q.rej_list = S->rej;
+  Q.cell_base = (int)base;
+  Q.counters = S->cnt;
+  Q.audit_cap = S->aud_cap;
+  const double thr = h->keys.surr_audit_frac * 4294967296.0;
+  Q.audit_thresh = thr >= 4294967295.0 ? 0xFFFFFFFFu : (unsigned int)thr;
+  Q.audit_seed = (unsigned int)h->keys.surr_audit_seed;
+  Q.aud_in = S->aud_in;
+  Q.aud_out = S->aud_out;
+  Q.aud_cell = S->aud_cell;
+  Q.aud_dt = S->aud_dt;
+  Q.Tstar = S->Tstar + base;
+  Q.scale = S->scale + base * (N + 1);
+  HIPCHK(h->var->trust(Q, gblocks, block, shmem, st));
+  return 0;
+}
+
+// The audit batch: a 1D box over the audit buffers (acap cells, component-major), each cell
+// with the dt_react of the cell it re-solves.
+static prk::BoxDesc audit_desc(pr_handle* h) {
+  Surrogate* S = h->surr;
+  const int N = h->mech->nsp;
+  const long c = S->aud_cap;
+  double* a_rY = S->aud_box;
+  double* a_src = a_rY + (long)N * c;
+  double* a_T = a_src + (long)N * c;
+  double* a_rH = a_T + c;
+  double* a_rHsrc = a_rH + c;
+  double* a_FC = a_rHsrc + c;
+  auto arr1 = [&](double* p, long ns) {
+    prk::Arr r;
+    r.p = p;
+    r.js = c;
+    r.ks = c;
+    r.ns = ns;
+    r.bx = r.by = r.bz = 0;
+    return r;
+  };
+  prk::BoxDesc a{};
+  a.nx = (int)c;
+  a.ny = 1;
+  a.ncells = c;
+  a.rY = arr1(a_rY, c);
+  a.rYsrc = arr1(a_src, c);
+  a.T = arr1(a_T, 0);
+  a.rH = arr1(a_rH, 0);
+  a.rHsrc = arr1(a_rHsrc, 0);
+  a.FC = arr1(a_FC, 0);
+  a.dt = 0.0;
+  a.dtv = S->aud_dt;
+  return a;
+}
+
+// Re-solves the audited cells with the BDF (queue and counters in the second half of
+// d_counters) and compares them with the learned result.
+static int audit_launch(pr_handle* h, const prk::KParams& P, const prk::BoxDesc& adesc, const prk::BoxDesc* dbox,
+                        const long* dstart, hipStream_t st) {
+  Surrogate* S = h->surr;
+  const int N = h->mech->nsp;
+  const unsigned int acap = S->aud_cap;
+  auto* base = static_cast<unsigned char*>(h->d_counters);
+  const unsigned grid = (unsigned)((acap + 127) / 128);
+  hipLaunchKernelGGL(prk::audit_unpack_kernel, dim3(grid), dim3(128), 0, st, N, &S->cnt[1], acap, S->aud_in,
+                     adesc.rY.p, adesc.rYsrc.p, adesc.T.p, adesc.rH.p, adesc.rHsrc.p, adesc.FC.p);
+  HIPCHK(hipGetLastError());
+  prk::KParams A = P;
+  A.one = adesc;
+  A.ncells = acap;
+  A.boxes = dbox;
+  A.box_start = dstart;
+  A.nbox = 1;
+  A.cell_list = nullptr;
+  A.ncells_dev = &S->cnt[1];
+  A.queue = reinterpret_cast<unsigned int*>(base + 128);
+  A.nfail = reinterpret_cast<unsigned int*>(base + 132);
+  A.stats = reinterpret_cast<unsigned long long*>(base + 192);
+  if (int rc = bdf_launch(h, A, acap, st)) return rc;
+  hipLaunchKernelGGL(prk::audit_err_kernel, dim3(grid), dim3(128), 0, st, N, &S->cnt[1], acap, adesc.rY.p,
+                     adesc.T.p, S->aud_out, S->aud_err);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Accepted / rejected / audited counts and the worst audit error of the last learned-map pass.
+static int surr_collect(pr_handle* h, unsigned int acap, hipStream_t st) {
+  Surrogate* S = h->surr;
+  unsigned int sc[3] = {0u, 0u, 0u};
+  HIPCHK(hipMemcpyAsync(sc, S->cnt, sizeof(sc), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  h->last.sur_accepted = sc[2];
+  h->last.sur_rejected = sc[0];
+  h->last.sur_audited = sc[1] < acap ? sc[1] : acap;
+  if (h->last.sur_audited > 0) {
+    std::vector<double> err(2 * (size_t)h->last.sur_audited);
+    unsigned int afail = 0;
+    auto* base = static_cast<unsigned char*>(h->d_counters);
+    HIPCHK(hipMemcpy(err.data(), S->aud_err, sizeof(double) * err.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&afail, base + 132, sizeof(afail), hipMemcpyDeviceToHost));
+    double mT = 0.0, mY = 0.0;
+    for (size_t a = 0; a < err.size(); a += 2) {
+      mT = err[a] > mT || err[a] != err[a] ? err[a] : mT;
+      mY = err[a + 1] > mY || err[a + 1] != err[a + 1] ? err[a + 1] : mY;
+    }
+    h->last.audit_max_dT_rel = mT;
+    h->last.audit_max_dY = mY;
+    h->last.audit_failed = afail;
+  }
+  return 0;
+}
+
 int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY,
                  const pr_array4* rYsrc, const pr_array4* T, const pr_array4* rH,
                  const pr_array4* rHsrc, const pr_array4* FC, const pr_iarray4* mask,
@@ -839,85 +1055,10 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   if (S) {
     int rc = ensure_surr_buffers(h, ncells);
     if (rc) return rc;
-    const int G = h->var->G;
-    const int block = pick_block(h);
-    const size_t shmem = (size_t)h->blob_bytes + (size_t)(block / G) * h->var->scratch;
-    int dev_cus = 0;
-    HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, h->device));
-    const long want = (ncells + (block / G) - 1) / (block / G);
-    const int gblocks = (int)(want < 8L * dev_cus ? want : 8L * dev_cus);
-    prk::PrepParams F{};
-    F.lo0 = lo[0];
-    F.lo1 = lo[1];
-    F.lo2 = lo[2];
-    F.nx = (int)nx;
-    F.ny = (int)ny;
-    F.ncells = ncells;
-    F.rY = P.one.rY;
-    F.rYsrc = P.one.rYsrc;
-    F.T = P.one.T;
-    F.rH = P.one.rH;
-    F.rHsrc = P.one.rHsrc;
-    F.mask = P.one.mask;
-    F.has_mask = P.one.has_mask;
-    F.clean = P.clean;
-    F.dt = dt;
-    F.xmu = S->xmu;
-    F.xsd = S->xsd;
-    F.X = S->X;
-    F.Tstar = S->Tstar;
-    F.scale = S->scale;
-    F.blob = h->d_blob;
-    F.blob_bytes = h->blob_bytes;
-    F.neff = h->mech->neff;
-    F.ninc = h->mech->ninc;
-    HIPCHK(h->var->prep(F, gblocks, block, shmem, st));
+    if ((rc = surr_prep(h, P.one, 0, st))) return rc;
     if ((rc = mlp_forward(h, ncells, st))) return rc;
     HIPCHK(hipMemsetAsync(S->cnt, 0, 8 * sizeof(unsigned int), st));
-    prk::TrustParams Q{};
-    Q.lo0 = lo[0];
-    Q.lo1 = lo[1];
-    Q.lo2 = lo[2];
-    Q.nx = (int)nx;
-    Q.ny = (int)ny;
-    Q.ncells = ncells;
-    Q.rY = P.one.rY;
-    Q.rYsrc = P.one.rYsrc;
-    Q.T = P.one.T;
-    Q.rH = P.one.rH;
-    Q.rHsrc = P.one.rHsrc;
-    Q.FC = P.one.FC;
-    Q.mask = P.one.mask;
-    Q.has_mask = P.one.has_mask;
-    Q.clean = P.clean;
-    Q.status = status_out;
-    Q.dt = dt;
-    Q.Yout = S->Yout;
-    Q.ymu = S->ymu;
-    Q.ysd = S->ysd;
-    Q.yeps = S->yeps;
-    Q.Mel = S->Mel;
-    Q.Minv = S->Minv;
-    Q.E = S->E;
-    Q.trust_rtol = h->keys.surr_trust_rtol;
-    Q.trust_atol = h->keys.surr_trust_atol;
-    Q.elem_tol = h->keys.surr_elem_tol;
-    Q.blob = h->d_blob;
-    Q.blob_bytes = h->blob_bytes;
-    Q.neff = h->mech->neff;
-    Q.ninc = h->mech->ninc;
-    Q.rej_list = S->rej;
-    Q.counters = S->cnt;
-    Q.audit_cap = S->aud_cap;
-    const double thr = h->keys.surr_audit_frac * 4294967296.0;
-    Q.audit_thresh = thr >= 4294967295.0 ? 0xFFFFFFFFu : (unsigned int)thr;
-    Q.audit_seed = (unsigned int)h->keys.surr_audit_seed;
-    Q.aud_in = S->aud_in;
-    Q.aud_out = S->aud_out;
-    Q.aud_cell = S->aud_cell;
-    Q.Tstar = S->Tstar;
-    Q.scale = S->scale;
-    HIPCHK(h->var->trust(Q, gblocks, block, shmem, st));
+    if ((rc = surr_trust(h, P.one, 0, st))) return rc;
     P.cell_list = S->rej;
     P.ncells_dev = &S->cnt[0];
   }
@@ -926,39 +1067,12 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   const unsigned int acap = S ? S->aud_cap : 0u;
   std::vector<prk::BoxDesc> descs(1, P.one);
   std::vector<long> starts = {0L, ncells};
-  prk::BoxDesc adesc = P.one;
+  prk::BoxDesc adesc{};
   if (S && acap > 0) {
-    const long c = acap;
-    double* a_rY = S->aud_box;
-    double* a_src = a_rY + (long)N * c;
-    double* a_T = a_src + (long)N * c;
-    double* a_rH = a_T + c;
-    double* a_rHsrc = a_rH + c;
-    double* a_FC = a_rHsrc + c;
-    auto arr1 = [&](double* p, long ns) {
-      prk::Arr r;
-      r.p = p;
-      r.js = c;
-      r.ks = c;
-      r.ns = ns;
-      r.bx = r.by = r.bz = 0;
-      return r;
-    };
-    adesc.lo0 = adesc.lo1 = adesc.lo2 = 0;
-    adesc.nx = (int)c;
-    adesc.ny = 1;
-    adesc.ncells = c;
-    adesc.rY = arr1(a_rY, c);
-    adesc.rYsrc = arr1(a_src, c);
-    adesc.T = arr1(a_T, 0);
-    adesc.rH = arr1(a_rH, 0);
-    adesc.rHsrc = arr1(a_rHsrc, 0);
-    adesc.FC = arr1(a_FC, 0);
-    adesc.has_mask = 0;
-    adesc.status = nullptr;
+    adesc = audit_desc(h);
     descs.push_back(adesc);
     starts.push_back(0L);
-    starts.push_back(c);
+    starts.push_back((long)acap);
   }
   const prk::BoxDesc* dboxes = nullptr;
   const long* dstart = nullptr;
@@ -998,39 +1112,16 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   }
   HIPCHK(hipEventRecord(h->ev2, st));
   if (S && acap > 0) {
-    // audit: the BDF re-solves the sampled accepted cells from their inputs, as the 1D box
-    // [1] of the table, with its own queue and counters (second half of d_counters)
-    const long c = acap;
-    const unsigned grid = (unsigned)((c + 127) / 128);
-    hipLaunchKernelGGL(prk::audit_unpack_kernel, dim3(grid), dim3(128), 0, st, N, &S->cnt[1], acap, S->aud_in,
-                       adesc.rY.p, adesc.rYsrc.p, adesc.T.p, adesc.rH.p, adesc.rHsrc.p, adesc.FC.p);
-    HIPCHK(hipGetLastError());
-    prk::KParams A = P;
-    A.one = adesc;
-    A.ncells = c;
-    A.boxes = dboxes + 1;
-    A.box_start = dstart + 2;
-    A.nbox = 1;
-    A.cell_list = nullptr;
-    A.ncells_dev = &S->cnt[1];
-    A.queue = reinterpret_cast<unsigned int*>(base + 128);
-    A.nfail = reinterpret_cast<unsigned int*>(base + 132);
-    A.stats = reinterpret_cast<unsigned long long*>(base + 192);
-    {
-      int rc = bdf_launch(h, A, c, st);
-      if (rc) return rc;
-    }
-    hipLaunchKernelGGL(prk::audit_err_kernel, dim3(grid), dim3(128), 0, st, N, &S->cnt[1], acap, adesc.rY.p,
-                       adesc.T.p, S->aud_out, S->aud_err);
-    HIPCHK(hipGetLastError());
+    // audit: the BDF re-solves the sampled accepted cells from their inputs, as the 1D box [1]
+    // of the table
+    int rc = audit_launch(h, P, adesc, dboxes + 1, dstart + 2, st);
+    if (rc) return rc;
   }
   HIPCHK(hipEventRecord(h->ev3, st));
   unsigned long long hs[8];
   unsigned int nfail = 0;
-  unsigned int sc[3] = {0u, 0u, 0u};
   HIPCHK(hipMemcpyAsync(hs, P.stats, sizeof(hs), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&nfail, P.nfail, sizeof(nfail), hipMemcpyDeviceToHost, st));
-  if (S) HIPCHK(hipMemcpyAsync(sc, S->cnt, sizeof(sc), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   float ms_all = 0.f, ms_sur = 0.f, ms_bdf = 0.f;
   HIPCHK(hipEventElapsedTime(&ms_all, h->ev0, h->ev3));
@@ -1048,25 +1139,14 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   h->last.kernel_ms = ms_bdf;
   h->last.total_ms = ms_all;
   h->last.surrogate_ms = ms_sur;
-  h->last.sur_accepted = sc[2];
-  h->last.sur_rejected = sc[0];
-  h->last.sur_audited = sc[1] < acap ? sc[1] : acap;
-  if (S && h->last.sur_audited > 0) {
-    float ms_aud = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms_aud, h->ev2, h->ev3));
-    h->last.audit_ms = ms_aud;
-    std::vector<double> err(2 * (size_t)h->last.sur_audited);
-    unsigned int afail = 0;
-    HIPCHK(hipMemcpy(err.data(), S->aud_err, sizeof(double) * err.size(), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(&afail, base + 132, sizeof(afail), hipMemcpyDeviceToHost));
-    double mT = 0.0, mY = 0.0;
-    for (size_t a = 0; a < err.size(); a += 2) {
-      mT = err[a] > mT || err[a] != err[a] ? err[a] : mT;
-      mY = err[a + 1] > mY || err[a + 1] != err[a + 1] ? err[a + 1] : mY;
+  if (S) {
+    int rc = surr_collect(h, acap, st);
+    if (rc) return rc;
+    if (h->last.sur_audited > 0) {
+      float ms_aud = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms_aud, h->ev2, h->ev3));
+      h->last.audit_ms = ms_aud;
     }
-    h->last.audit_max_dT_rel = mT;
-    h->last.audit_max_dY = mY;
-    h->last.audit_failed = afail;
   }
   if (time) *time += dt;
   return (int)nfail;
@@ -1142,9 +1222,37 @@ int pr_react_batch_enqueue(pr_handle* h, int nbox, const pr_box* boxes, void* st
     int rc = autotune(h, P, descs[big], st);
     if (rc) return rc;
   }
+  // With a learned map loaded: every box's cells through the map (features per box, one network
+  // pass over all boxes, trust check per box with its own dt), then one persistent BDF launch
+  // over the rejected cells of all boxes. No memset and no allocation once buffers are sized, so
+  // the sequence can be captured in a hipGraph.
+  Surrogate* S = (h->surr && h->keys.surr_enable) ? h->surr : nullptr;
+  h->batch_surr = 0;
+  h->batch_acap = 0;
+  if (S) {
+    int rc = ensure_surr_buffers(h, total);
+    if (rc) return rc;
+    for (int b = 0; b < nbox; ++b)
+      if ((rc = surr_prep(h, descs[(size_t)b], starts[(size_t)b], st))) return rc;
+    if ((rc = mlp_forward(h, total, st))) return rc;
+    hipLaunchKernelGGL(prk::zero_u32, dim3(1), dim3(64), 0, st, S->cnt, 8);
+    HIPCHK(hipGetLastError());
+    for (int b = 0; b < nbox; ++b)
+      if ((rc = surr_trust(h, descs[(size_t)b], starts[(size_t)b], st))) return rc;
+    P.cell_list = S->rej;
+    P.ncells_dev = &S->cnt[0];
+  }
+  prk::BoxDesc adesc{};
+  const bool audit = S && S->aud_cap > 0;
+  if (audit) {  // the audit batch: box [nbox] of the table, with its own start pair
+    adesc = audit_desc(h);
+    descs.push_back(adesc);
+    starts.push_back(0L);
+    starts.push_back((long)S->aud_cap);
+  }
+  const prk::BoxDesc* dboxes = nullptr;
+  const long* dstart = nullptr;
   {
-    const prk::BoxDesc* dboxes = nullptr;
-    const long* dstart = nullptr;
     int rc = stage_boxes(h, descs, starts, st, &dboxes, &dstart);
     if (rc) return rc;
     P.boxes = dboxes;
@@ -1155,6 +1263,12 @@ int pr_react_batch_enqueue(pr_handle* h, int nbox, const pr_box* boxes, void* st
     int rc = bdf_launch(h, P, total, st);
     if (rc) return rc;
   }
+  if (audit) {
+    int rc = audit_launch(h, P, adesc, dboxes + nbox, dstart + nbox + 1, st);
+    if (rc) return rc;
+  }
+  h->batch_surr = S ? 1 : 0;
+  h->batch_acap = S ? S->aud_cap : 0u;  // read after ensure_surr_buffers has sized the audit
   return 0;
 }
 
@@ -1184,6 +1298,10 @@ int pr_collect(pr_handle* h, void* stream) {
   h->last.netf = (int64_t)hs[6];
   h->last.ncfn = (int64_t)hs[7];
   h->last.failed = nfail;
+  if (h->batch_surr && h->surr) {
+    int rc = surr_collect(h, h->batch_acap, st);
+    if (rc) return rc;
+  }
   return (int)nfail;
 }
 

@@ -203,12 +203,14 @@ struct TrustParams {
   double trust_rtol, trust_atol, elem_tol;
   const unsigned char* blob;
   int blob_bytes, neff, ninc;
-  int* rej_list;
+  int* rej_list;           // rejected cells as cell_base + box-local index
+  int cell_base;
   unsigned int* counters;  // [0] rejected, [1] audited, [2] accepted
   unsigned int audit_cap, audit_thresh, audit_seed;
   double* aud_in;   // [cap x (2 N + 3)] rhoY0, src, T0, rhoh0, src_h
   double* aud_out;  // [cap x (N + 1)] surrogate rhoY1, T1
   int* aud_cell;
+  double* aud_dt;   // [cap] the audited cell's dt_react
 };
 
 // network output -> increment: u = o sd + mu; d = u (scale + eps) with the cell's rate scale
@@ -382,6 +384,7 @@ __global__ void __launch_bounds__(BLOCK) trust_kernel(TrustParams P) {
             ai[2 * N + 1] = rhoh0;
             ai[2 * N + 2] = srch;
             P.aud_cell[slot] = (int)c;
+            P.aud_dt[slot] = dt;
           }
         }
       }
@@ -399,7 +402,7 @@ __global__ void __launch_bounds__(BLOCK) trust_kernel(TrustParams P) {
       }
     } else if (lane == 0) {
       slot = atomicAdd(&P.counters[0], 1u);
-      P.rej_list[slot] = (int)c;
+      P.rej_list[slot] = P.cell_base + (int)c;
     }
   }
 }

@@ -124,3 +124,106 @@ def test_load_rejects_bad_networks(gpu):
         xn[1, 3] = 0.0
         r.load_surrogate([N + 2, 8, N + 1], np.zeros((N + 2) * 8 + 8 + 8 * (N + 1) + N + 1), xn, np.ones((3, N + 1)))
     r.close()
+
+
+def _batch_boxes():
+    # mixed sizes, dt_react (the network saw 5e-6 only) and a mask
+    specs = [(32, 8, 4, 5e-6, 51), (16, 16, 2, 2e-6, 52), (8, 8, 8, 5e-6, 53), (64, 2, 2, 4e-6, 54)]
+    boxes = []
+    for nx, ny, nz, dt, seed in specs:
+        b = W.ch4_air_flame_box(nx=nx, ny=ny, nz=nz, seed=seed, dt=dt)
+        if seed == 52:
+            b["mask"][::7] = -1
+        boxes.append(b)
+    return boxes
+
+
+def test_batch_with_learned_map_matches_single_boxes(gpu):
+    """pr_react_batch with a learned map: every box's cells go through the map (one network pass
+    over all boxes), the rejected cells of all boxes through one BDF launch; box by box the bits
+    equal pr_react_box with the same map, and the accepted / rejected / audited counts add up."""
+    import torch
+    from pelelmex_amd import surrogate as sg
+    from pelelmex_amd.reactor import Reactor, box_tensors
+    model = _model()
+    boxes = _batch_boxes()
+    keys = _keys(model, **{"surrogate.audit_frac": 0.05})
+    r = Reactor.create("ReactorCvode", "drm19", keys)
+    r.init(2, 1)
+    sg.attach(r, model)
+    base = [box_tensors(b, gpu) for b in boxes]
+    singles, acc, rej, aud = [], 0, 0, 0
+    for b, t0 in zip(boxes, base):
+        t = {k: v.clone() for k, v in t0.items()}
+        status = torch.full((b["T"].size,), 7, dtype=torch.int32, device=gpu)
+        nf, _, _ = r.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], t["mask"], b["dt"], 0.0,
+                           status=status)
+        assert nf == 0
+        st = r.stats()
+        acc, rej, aud = acc + st["sur_accepted"], rej + st["sur_rejected"], aud + st["sur_audited"]
+        singles.append((t, status))
+    assert acc > 0 and rej > 0
+    batch = []
+    for b, t0 in zip(boxes, base):
+        t = {k: v.clone() for k, v in t0.items()}
+        t["status"] = torch.full((b["T"].size,), 7, dtype=torch.int32, device=gpu)
+        t["dt"] = b["dt"]
+        batch.append(t)
+    assert r.react_batch(batch) == 0
+    st = r.stats()
+    torch.cuda.synchronize()
+    r.close()
+    assert (st["sur_accepted"], st["sur_rejected"], st["sur_audited"]) == (acc, rej, aud)
+    assert st["audit_max_dT_rel"] <= 1e-4
+    for (ts, ss), tb in zip(singles, batch):
+        for k in ("rY", "T", "rH", "FC"):
+            np.testing.assert_array_equal(tb[k].cpu().numpy(), ts[k].cpu().numpy())
+        np.testing.assert_array_equal(tb["status"].cpu().numpy(), ss.cpu().numpy())
+
+
+def test_batch_with_learned_map_graph_replay(gpu):
+    """The learned map + batched fallback captured in one hipGraph (torch.cuda.graph around
+    pr_react_batch_enqueue) and replayed three times, inputs restored eagerly and the stream
+    synchronized between replays: every replay gives the eager call's bits and counts."""
+    import torch
+    from pelelmex_amd import surrogate as sg
+    from pelelmex_amd.reactor import Reactor, box_tensors
+    model = _model()
+    boxes = _batch_boxes()
+    r = Reactor.create("ReactorCvode", "drm19", _keys(model, **{"surrogate.audit_frac": 0.05}))
+    r.init(2, 1)
+    sg.attach(r, model)
+    base = [box_tensors(b, gpu) for b in boxes]
+    work = []
+    for b, t0 in zip(boxes, base):
+        t = {k: v.clone() for k, v in t0.items()}
+        t["dt"] = b["dt"]
+        work.append(t)
+
+    def restore():
+        for t, t0 in zip(work, base):
+            for k in ("rY", "T", "rH", "FC"):
+                t[k].copy_(t0[k])
+
+    restore()
+    assert r.react_batch(work) == 0  # eager reference; sizes every buffer the capture uses
+    st0 = r.stats()
+    ref = [{k: t[k].clone() for k in ("rY", "T", "rH", "FC")} for t in work]
+    restore()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        r.react_batch(work, enqueue_only=True)
+    for rep in range(3):
+        restore()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert r.collect() == 0
+        st = r.stats()
+        assert (st["sur_accepted"], st["sur_rejected"], st["sur_audited"]) == \
+            (st0["sur_accepted"], st0["sur_rejected"], st0["sur_audited"])
+        for t, rf in zip(work, ref):
+            for k in rf:
+                np.testing.assert_array_equal(t[k].cpu().numpy(), rf[k].cpu().numpy(), err_msg=f"replay {rep} {k}")
+    r.close()
