@@ -161,15 +161,25 @@ class MLP:
         self.refresh_bf16()
 
 
-def generate_pairs(reactor, seeds, dts, nx=64, device="cuda", log=print):
-    """(state, result) pairs from the GPU BDF on flame-box states without external sources.
-    Returns device tensors rY0 [N, P], T0 [P], dt [P], rY1 [N, P], T1 [P] and the BDF's right-hand
-    side evaluations per cell nfe [P] (its cost)."""
+def workload_box(mech, seed, dt, nx=64, src_frac=0.0):
+    """A training or calibration box of the mechanism's workload: the drm19 flame-like box
+    (optionally with sources), or a syn53 60 atm box (no sources, no mask)."""
+    if mech == "drm19":
+        return workloads.ch4_air_flame_box(nx=nx, ny=nx, nz=nx, seed=seed, dt=dt, src_frac=src_frac)
+    if mech == "syn53":
+        return workloads.syn53_chunk(0, nx=nx, ny=nx, nz=nx, seed=seed, dt=dt, masked=0.0)
+    raise ValueError(f"no training workload for mechanism {mech}")
+
+
+def generate_pairs(reactor, seeds, dts, nx=64, device="cuda", log=print, mech="drm19"):
+    """(state, result) pairs from the GPU BDF on the mechanism's workload states without external
+    sources. Returns device tensors rY0 [N, P], T0 [P], dt [P], rY1 [N, P], T1 [P] and the BDF's
+    right-hand side evaluations per cell nfe [P] (its cost)."""
     import torch
     from .reactor import box_tensors
     out = {k: [] for k in ("rY0", "T0", "rH0", "dt", "rY1", "T1", "nfe")}
     for seed, dt in zip(seeds, dts):
-        box = workloads.ch4_air_flame_box(nx=nx, ny=nx, nz=nx, seed=seed, dt=dt, src_frac=0.0)
+        box = workload_box(mech, seed, dt, nx=nx)
         t = box_tensors(box, device)
         rY0, T0 = t["rY"].reshape(reactor.nsp, -1).clone(), t["T"].reshape(-1).clone()
         rH0 = t["rH"].reshape(-1).clone()
@@ -307,7 +317,8 @@ WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "surrogate_dr
 
 
 def calibrate(reactor_keys, model, seeds=(40100, 40101), target=1e-6, nx=64, device="cuda", log=print,
-              grid_rtol=(1e-1, 3e-2, 1e-2, 3e-3, 1e-3, 3e-4, 1e-4), grid_atol=(1e-9, 3e-10, 1e-10, 3e-11, 1e-11)):
+              grid_rtol=(1e-1, 3e-2, 1e-2, 3e-3, 1e-3, 3e-4, 1e-4), grid_atol=(1e-9, 3e-10, 1e-10, 3e-11, 1e-11),
+              mech="drm19", dts=(5e-6,)):
     """Trust tolerances for a trained network. On held-out flame boxes with sources (seeds),
     pr_react_box runs the learned path with each (trust_rtol, trust_atol) of the grid and its
     accepted cells (FC = 1) are compared with a BDF-only call. Among the pairs whose accepted
@@ -319,7 +330,7 @@ def calibrate(reactor_keys, model, seeds=(40100, 40101), target=1e-6, nx=64, dev
     from .reactor import box_tensors
 
     def run(t, pr, box, keys, with_model):
-        rr = Reactor.create("ReactorCvode", "drm19", keys)
+        rr = Reactor.create("ReactorCvode", mech, keys)
         rr.init(2, 1)
         if with_model:
             attach(rr, model)
@@ -332,8 +343,8 @@ def calibrate(reactor_keys, model, seeds=(40100, 40101), target=1e-6, nx=64, dev
         return t["T"].reshape(-1).clone(), t["FC"].reshape(-1).clone()
 
     results = {}
-    for seed in seeds:
-        box = workloads.ch4_air_flame_box(nx=nx, ny=nx, nz=nx, seed=seed)
+    for i, seed in enumerate(seeds):
+        box = workload_box(mech, seed, dts[i % len(dts)], nx=nx, src_frac=0.05)
         t = box_tensors(box, device)
         pr = {k: t[k].clone() for k in ("rY", "T", "rH")}
         T_ref, FC_ref = run(t, pr, box, reactor_keys, False)
@@ -350,6 +361,8 @@ def calibrate(reactor_keys, model, seeds=(40100, 40101), target=1e-6, nx=64, dev
                 r["accepted"].append(float(a.double().mean()))
                 r["bdf_work_share"].append(float(fc[a].sum() / fc.sum()))
                 r["max_errT"] = max(r["max_errT"], e)
+                log(f"  calibration box {seed}: trust_rtol {rt:g} trust_atol {at:g} accepted {float(a.double().mean()):.4f} "
+                    f"max dT/T {e:.2e}")
     sweep = []
     for r in results.values():
         r["accepted"] = float(np.mean(r["accepted"]))
@@ -366,8 +379,9 @@ def calibrate(reactor_keys, model, seeds=(40100, 40101), target=1e-6, nx=64, dev
 
 
 def main():
-    """Trains the drm19 network on the GPU and calibrates its trust tolerances:
-    python -m pelelmex_amd.surrogate --out <path.npz> [--boxes 8 --epochs 20]."""
+    """Trains a network on the GPU and calibrates its trust tolerances:
+    python -m pelelmex_amd.surrogate --out <path.npz> [--mech drm19 --boxes 8 --epochs 20
+    --hidden 256,256,256,256 --dts 5e-6]."""
     import argparse
     import json
     import torch
@@ -381,15 +395,20 @@ def main():
     ap.add_argument("--seed", type=int, default=40040)
     ap.add_argument("--target", type=float, default=1e-6)
     ap.add_argument("--model", default=None, help="calibrate this trained network instead of training one")
+    ap.add_argument("--mech", default="drm19", choices=["drm19", "syn53"])
+    ap.add_argument("--hidden", default="256,256,256,256", help="hidden layer widths")
+    ap.add_argument("--dts", default="5e-6", help="dt_react values, cycled over the training and calibration boxes")
     args = ap.parse_args()
     log = lambda *a: print(*a, flush=True)
+    hidden = tuple(int(x) for x in args.hidden.split(","))
+    dts = [float(x) for x in args.dts.split(",")]
     keys = {"ode.rtol": 1e-10, "ode.atol": 1e-10, "cvode.max_order": 4, "cvode.max_substeps": 10000}
-    r = Reactor.create("ReactorCvode", "drm19", keys)
+    r = Reactor.create("ReactorCvode", args.mech, keys)
     r.init(2, 1)
     dev = torch.device("cuda:0")
     if args.model:
         m = load(args.model)
-        best, sweep = calibrate(keys, m, target=args.target, nx=args.nx, device=dev, log=log)
+        best, sweep = calibrate(keys, m, target=args.target, nx=args.nx, device=dev, log=log, mech=args.mech, dts=dts)
         meta = {k: v for k, v in m.items() if k not in ("dims", "params", "xnorm", "ynorm", "trust_rtol", "trust_atol",
                                                        "calibration")}
         meta["calibration"] = json.dumps({"best": best, "sweep": sweep, "target": args.target})
@@ -399,17 +418,19 @@ def main():
         np.savez(args.out, dims=m["dims"], params=m["params"], xnorm=m["xnorm"], ynorm=m["ynorm"], **meta)
         log(f"saved {args.out}")
         return
-    pairs = generate_pairs(r, [args.seed + i for i in range(args.boxes)], [5e-6] * args.boxes, nx=args.nx,
-                           device=dev, log=log)
+    pairs = generate_pairs(r, [args.seed + i for i in range(args.boxes)], [dts[i % len(dts)] for i in range(args.boxes)],
+                           nx=args.nx, device=dev, log=log, mech=args.mech)
     _, _, ukeep = scaled_targets(r, pairs, umax=args.umax)
     keep = (pairs["nfe"] <= args.nfe_max) & ukeep
     log(f"pairs {pairs['T0'].numel()}, training on {int(keep.sum())} (BDF cost <= {args.nfe_max} RHS, |u| <= {args.umax})")
     sub = select_pairs(pairs, keep)
     del pairs
-    mlp, xn, yn, hist = train(r, sub, epochs=args.epochs, log=log)
-    best, sweep = calibrate(keys, (mlp, xn, yn), target=args.target, nx=args.nx, device=dev, log=log)
+    mlp, xn, yn, hist = train(r, sub, hidden=hidden, epochs=args.epochs, log=log)
+    best, sweep = calibrate(keys, (mlp, xn, yn), target=args.target, nx=args.nx, device=dev, log=log, mech=args.mech,
+                            dts=dts)
     meta = {"loss": np.asarray(hist, np.float64), "train_pairs": int(keep.sum()), "train_seed": args.seed,
             "boxes": args.boxes, "epochs": args.epochs, "nfe_max": args.nfe_max, "umax": args.umax,
+            "mech": args.mech, "dts": np.asarray(dts, np.float64),
             "calibration": json.dumps({"best": best, "sweep": sweep, "target": args.target})}
     if best is not None:
         meta["trust_rtol"] = best["trust_rtol"]
