@@ -110,13 +110,17 @@ typedef struct {
 
 /* Integrates every box of the list over its own dt_react with the semantics of pr_react_box,
    in one persistent launch whose work queue holds the cells of all boxes (so boxes of mixed
-   sizes and stiffness balance across the GPU). The learned map is not used on this path.
-   Returns the number of failed cells (>= 0) or a negative error code. */
+   sizes and stiffness balance across the GPU). With a learned map loaded (and
+   surrogate.enable = 1) every box's cells go through it first - features per box, one network
+   pass over all boxes, the FP64 trust check per box with its dt - and the persistent launch
+   integrates the rejected cells of all boxes; results are those of pr_react_box with the same
+   map, box by box. Returns the number of failed cells (>= 0) or a negative error code. */
 int pr_react_batch(pr_handle* h, int nbox, const pr_box* boxes, void* stream);
 /* pr_react_batch in two halves, for callers that capture the work in a hipGraph: _enqueue only
    enqueues on `stream` (a copy of the box table from the handle's pinned staging memory, the
-   counter reset kernel and the integration kernel: no synchronization, no memset node, and no
-   allocation once a first call of the same or larger size has run); pr_collect waits for the
+   counter reset kernels, the learned map's kernels when one is loaded, the integration and
+   audit kernels: no synchronization, no memset node, and no allocation once a first call of
+   the same or larger size has run); pr_collect waits for the
    stream and returns the failed-cell count of the last integration, filling pr_get_stats. A
    captured batch replays the box table staged at capture; other react calls on the same handle
    restage it. */
