@@ -32,6 +32,8 @@
  *   pr_set_temperature_box  PeleLM::setTemperature per box (Source/PeleLMeX_Eos.cpp:218-240,
  *                    getTfromHY Source/PeleLMeX_K.H:700-723)
  *   pr_floor_species_box    PeleLM::floorSpecies per box (Source/PeleLMeX_Utils.cpp:649-690)
+ *   pr_reaction_source_box  I_R formation in advanceChemistry (Source/PeleLMeX_Reactions.cpp:141-160,
+ *                    342-347)
  *   pr_heat_release_box     PeleLM::getHeatRelease per box (Source/PeleLMeX_Reactions.cpp:477-506,
  *                    getHGivenT Source/PeleLMeX_K.H:598-615)
  *   pr_surrogate_load, pr_surrogate_features, pr_surrogate_forward, pr_mlp_*   no reference
@@ -167,6 +169,13 @@ int pr_floor_species_box(pr_handle* h, const int lo[3], const int hi[3], const p
                          const pr_array4* rho, const pr_array4* rH, const pr_array4* T, void* stream);
 int pr_heat_release_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* T,
                         const pr_array4* IR, const pr_array4* HR, void* stream);
+/* I_R formation after the advance: I_R_k = -(rhoY_old_k - rhoY_new_k) * (1 / dt) - F_rhoY_k over
+   the box (MKS Array4 views, NUM_SPECIES components each; the reciprocal is formed once, as
+   PeleLMeX forms dt_inv). Replaces the ParallelFor at Source/PeleLMeX_Reactions.cpp:141-160
+   (and the I_R lines of :342-347 on the chemistry BoxArray). Synchronous. */
+int pr_reaction_source_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY_old,
+                           const pr_array4* rY_new, const pr_array4* extF_rY, const pr_array4* I_R, double dt,
+                           void* stream);
 /* Learned dt_react map. Network: nlayers dense layers, dims[0] = NUM_SPECIES + 2 inputs
    (log10(Y_k + 1e-20), T / 2000, log10(dt)), dims[nlayers] = NUM_SPECIES + 1 outputs (dY_k, dT),
    GELU between layers. params (host, fp32): for each layer W [dims[l] x dims[l+1]] row-major then

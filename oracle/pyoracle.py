@@ -198,6 +198,15 @@ class Mech:
         return dict(rY=rY, T=T, rH=rH, FC=FC, status=status, nst=nst.value)
 
 
+def reaction_source(rY_old, rY_new, F, dt):
+    """I_R formation after the advance (Source/PeleLMeX_Reactions.cpp:141-160):
+    I_R = -(rhoY_old - rhoY_new) * dt_inv - F_rhoY with dt_inv = 1 / dt formed once, entrywise in
+    IEEE float64 in that order (numpy does not contract)."""
+    dt_inv = 1.0 / float(dt)
+    return -(np.asarray(rY_old, dtype=np.float64) - np.asarray(rY_new, dtype=np.float64)) * dt_inv \
+        - np.asarray(F, dtype=np.float64)
+
+
 def exp(x):
     return lib().orc_exp(float(x))
 

@@ -488,16 +488,17 @@ struct Cell {
   static constexpr int n = N + 1;
   static constexpr int RPL = (n + G - 1) / G;
   static constexpr int RXL = (R + G - 1) / G;
-  // Above 32 components (one row per lane of a 64-lane group) the LU factors stay in registers
-  // for the whole integration and the group's LDS holds no n x n area: the Jacobian is assembled
-  // in its global slot.
-  static constexpr bool lu_resident = (G == 64 && n > 32);
+  // Above 32 components (one row per lane of a 64-lane group) the group's LDS holds no n x n
+  // area: the LU factors live in the group's global slot, stored entry by entry with the 64
+  // lanes' rows side by side (entry j of lane l's row at j * 64 + l, so every access coalesces),
+  // and the Jacobian is assembled in the slot itself.
+  static constexpr bool lu_global = (G == 64 && n > 32);
   // Otherwise, with one cell per wavefront the saved Jacobian and its records sit in LDS after
   // the group's scratch (few groups per block); narrower groups keep them in a global slot.
-  static constexpr bool js_lds = (G == 64 && !lu_resident);
+  static constexpr bool js_lds = (G == 64 && !lu_global);
   static constexpr int lds_doubles =
-      Smem<N, R>::scratch_doubles + (lu_resident ? 0 : n * n) + (js_lds ? Smem<N, R>::slot_doubles : 0);
-  static constexpr int gslot_doubles = js_lds ? 0 : Smem<N, R>::slot_doubles;
+      Smem<N, R>::scratch_doubles + (lu_global ? 0 : n * n) + (js_lds ? Smem<N, R>::slot_doubles : 0);
+  static constexpr int gslot_doubles = js_lds ? 0 : Smem<N, R>::slot_doubles + (lu_global ? 64 * n : 0);
 };
 
 // ---------------------------------------------------------------- right-hand side
@@ -666,15 +667,15 @@ __device__ __forceinline__ void rate_rhoY_eval(const Smem<N, R>& S, int lane, co
 // ---------------------------------------------------------------- analytic Jacobian
 // Exact Jacobian of the right-hand side with T = y_T, written into S.Js (row-major). Rows are
 // assembled in LDS (the saved Jacobian itself at G = 64, else the LU area, which is dead here:
-// a factorization always follows) and then copied to the global slot; with register-resident
-// factors there is no LU area and they are assembled in the global slot itself (each lane
-// updates its own row; a wavefront's global accesses to one address complete in order).
+// a factorization always follows) and then copied to the global slot; above 32 components there
+// is no LU area in LDS and they are assembled in the global slot itself (each lane updates its
+// own row; a wavefront's global accesses to one address complete in order).
 template <int N, int R, int G>
 __device__ __forceinline__ void jac_eval(const Smem<N, R>& S, int lane, const double (&y)[Cell<N, R, G>::RPL],
                                          const double (&src)[Cell<N, R, G>::RPL], double rhohsrc) {
   constexpr int RPL = Cell<N, R, G>::RPL;
   constexpr int n = N + 1;
-  double* const A = (Cell<N, R, G>::js_lds || Cell<N, R, G>::lu_resident) ? S.Js : S.LU;  // assembly rows
+  double* const A = (Cell<N, R, G>::js_lds || Cell<N, R, G>::lu_global) ? S.Js : S.LU;  // assembly rows
   const double T = gbcast<G>(y[N / G], N % G);
   const double lnT = pr_log(T), invT = 1.0 / T, lnRTP = pr_log((PR_RU * T) / PR_PATM);
   double cc[RPL], cpk[RPL], dcpk[RPL];
@@ -779,7 +780,7 @@ __device__ __forceinline__ void jac_eval(const Smem<N, R>& S, int lane, const do
     }
   }
   group_sync();
-  if constexpr (!Cell<N, R, G>::js_lds && !Cell<N, R, G>::lu_resident) {
+  if constexpr (!Cell<N, R, G>::js_lds && !Cell<N, R, G>::lu_global) {
     for (int e = lane; e < n * n; e += G) S.Js[e] = A[e];
     group_sync();
   }
@@ -941,8 +942,8 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
   return singular;
 }
 
-// The LU rows persist in the group's LDS area between factorizations; registers hold them only
-// while a factorization or a solve runs.
+// The LU rows persist between factorizations in the group's LDS area (or, above 32 components,
+// in its global slot); registers hold them only while a factorization or a solve runs.
 // At 32 lanes (one row per lane) the rows are stored in pivot order (slot k = row pivoted at step
 // k), so that the solve finds the pivot row of step k in lane k.
 template <int N, int R, int G>
@@ -954,6 +955,12 @@ template <int N, int R, int G>
 __device__ __forceinline__ void store_lu(const Smem<N, R>& S, int lane, const double (&lu)[Cell<N, R, G>::RPL][N + 1],
                                          const int (&pst)[Cell<N, R, G>::RPL]) {
   constexpr int n = N + 1;
+  if constexpr (Cell<N, R, G>::lu_global) {
+#pragma unroll
+    for (int j = 0; j < n; ++j) S.LU[j * 64 + lane] = lu[0][j];  // lanes >= n: copies, never read
+    (void)pst;
+    return;
+  }
 #pragma unroll
   for (int s = 0; s < Cell<N, R, G>::RPL; ++s) {
     const int i = lane + G * s;
@@ -967,6 +974,11 @@ __device__ __forceinline__ void store_lu(const Smem<N, R>& S, int lane, const do
 template <int N, int R, int G>
 __device__ __forceinline__ void load_lu(const Smem<N, R>& S, int lane, double (&lu)[Cell<N, R, G>::RPL][N + 1]) {
   constexpr int n = N + 1;
+  if constexpr (Cell<N, R, G>::lu_global) {
+#pragma unroll
+    for (int j = 0; j < n; ++j) lu[0][j] = S.LU[j * 64 + lane];
+    return;
+  }
 #pragma unroll
   for (int s = 0; s < Cell<N, R, G>::RPL; ++s) {
     const int i = lane + G * s;

@@ -62,7 +62,7 @@ __device__ __forceinline__ double* setup_views(Smem<N, R>& S, unsigned char* sme
   S.sCpm = S.sHm + N;
   S.q = S.sCpm + N;
   S.perm = reinterpret_cast<int*>(S.q + R);
-  S.LU = Cell<N, R, G>::lu_resident ? nullptr : S.q + R + (n + 1) / 2 + 1;
+  S.LU = Cell<N, R, G>::lu_global ? nullptr : S.q + R + (n + 1) / 2 + 1;
   if constexpr (Cell<N, R, G>::js_lds) {
     S.Js = S.LU + n * n;
     S.jrec = S.Js + n * n;
@@ -70,6 +70,7 @@ __device__ __forceinline__ double* setup_views(Smem<N, R>& S, unsigned char* sme
     const long slot = (long)blockIdx.x * (blockDim.x / G) + grp;
     S.Js = gscr + slot * Cell<N, R, G>::gslot_doubles;
     S.jrec = S.Js + n * n;
+    if constexpr (Cell<N, R, G>::lu_global) S.LU = S.jrec + 4 * R;  // 64 n doubles after the records
   } else {
     S.Js = nullptr;  // kernels without a Jacobian
     S.jrec = nullptr;
@@ -112,8 +113,6 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
   int pst[RPL];
   double rinv[RPL];
   int permv = 0;
-  // the factors themselves when they stay in registers (more than 32 components at 64 lanes)
-  double luk[RPL][Cell<N, R, G>::lu_resident ? n : 1];
 #pragma unroll
   for (int s = 0; s < RPL; ++s) {
     int i = lane + G * s;
@@ -575,9 +574,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
         } else {
           c.jcur = 0;
         }
-        if constexpr (Cell<N, R, G>::lu_resident) {
-          singular = lu_factor<N, R, G>(S, lane, c.gamma, luk, pst, rinv, permv);
-        } else {
+        {
           double lu[RPL][n];
           singular = lu_factor<N, R, G>(S, lane, c.gamma, lu, pst, rinv, permv);
           store_lu<N, R, G>(S, lane, lu, pst);
@@ -596,9 +593,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
         double b[RPL], dlt[RPL];
 #pragma unroll
         for (int s = 0; s < RPL; ++s) b[s] = -res[s];
-        if constexpr (Cell<N, R, G>::lu_resident) {
-          lu_solve<N, R, G>(S, lane, luk, pst, rinv, permv, b, dlt);
-        } else {
+        {
           double lu[RPL][n];
           load_lu<N, R, G>(S, lane, lu);
           lu_solve<N, R, G>(S, lane, lu, pst, rinv, permv, b, dlt);

@@ -154,8 +154,8 @@ struct Variant {
 // latency bound). At 64 lanes the group's LDS does not fit 3 waves; at 16 lanes 4 cells share
 // a wave and registers already spill at 2. LiDryer at 16 lanes fits 3 waves without spills
 // (21% faster than 2 on a 32^3 box); at 8 lanes 4 waves spill and are slower. A 54-component
-// state (syn53) keeps its LU factors in registers (108 VGPRs): at 2 waves 128 VGPRs spill to
-// scratch memory, at 1 wave none (the AGPRs hold them).
+// state (syn53) keeps its LU factors in its global slot: 229 VGPRs at 2 waves (no spills), 52
+// spilled at 3.
 const Variant kVariants[] = {
     PR_VARIANT("LiDryer", PR_LIDRYER_NSP, PR_LIDRYER_NR, 16, 3),
     PR_VARIANT("LiDryer", PR_LIDRYER_NSP, PR_LIDRYER_NR, 8, 2),
@@ -163,7 +163,7 @@ const Variant kVariants[] = {
     PR_VARIANT("drm19", PR_DRM19_NSP, PR_DRM19_NR, 16, 2),
     PR_VARIANT("drm19", PR_DRM19_NSP, PR_DRM19_NR, 64, 2),
     PR_VARIANT("syn53", PR_SYN53_NSP, PR_SYN53_NR, 64, 2),
-    PR_VARIANT("syn53", PR_SYN53_NSP, PR_SYN53_NR, 64, 1),
+    PR_VARIANT("syn53", PR_SYN53_NSP, PR_SYN53_NR, 64, 3),
 };
 
 const Variant* find_variant(const char* mech, int G, int W = 0) {
@@ -1520,6 +1520,44 @@ int pr_heat_release_box(pr_handle* h, const int lo[3], const int hi[3], const pr
   P.IR = to_arr(IR);
   P.HR = to_arr(HR);
   return eos_launch(h, prk::EOS_HEAT_RELEASE, lo, hi, P, stream);
+}
+
+int pr_reaction_source_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY_old,
+                           const pr_array4* rY_new, const pr_array4* extF_rY, const pr_array4* I_R, double dt,
+                           void* stream) {
+  if (!h || !lo || !hi) return fail(PR_ERR_ARG, "NULL argument");
+  const int N = h->mech->nsp;
+  if (!arr_ok(rY_old, N) || !arr_ok(rY_new, N) || !arr_ok(extF_rY, N) || !arr_ok(I_R, N))
+    return fail(PR_ERR_ARG, "missing array");
+  for (const pr_array4* a : {rY_old, rY_new, extF_rY, I_R})
+    if (a->ncomp && a->ncomp < N) return fail(PR_ERR_ARG, "arrays need NUM_SPECIES components");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(PR_ERR_ARG, "dt must be positive and finite");
+  const long nx = (long)hi[0] - lo[0] + 1, ny = (long)hi[1] - lo[1] + 1, nz = (long)hi[2] - lo[2] + 1;
+  if (nx <= 0 || ny <= 0 || nz <= 0) return fail(PR_ERR_ARG, "empty box");
+  const long ncells = nx * ny * nz;
+  if (ncells > 0x7fffffffL) return fail(PR_ERR_ARG, "box too large");
+  HIPCHK(hipSetDevice(h->device));
+  prk::IrParams P{};
+  P.lo0 = lo[0];
+  P.lo1 = lo[1];
+  P.lo2 = lo[2];
+  P.nx = (int)nx;
+  P.ny = (int)ny;
+  P.ncells = ncells;
+  P.N = N;
+  P.rYo = to_arr(rY_old);
+  P.rYn = to_arr(rY_new);
+  P.F = to_arr(extF_rY);
+  P.IR = to_arr(I_R);
+  P.dt_inv = 1.0 / dt;
+  int dev_cus = 0;
+  HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, h->device));
+  const long want = (ncells * N + 255) / 256, cap = 16L * dev_cus;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(prk::ir_kernel, dim3((unsigned)(want < cap ? want : cap)), dim3(256), 0, st, P);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
 }
 
 int pr_surrogate_load(pr_handle* h, int nlayers, const int* dims, const float* params, const float* xnorm,

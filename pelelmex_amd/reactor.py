@@ -24,7 +24,7 @@ _LIB = None
 
 EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_react_batch", "pr_react_batch_enqueue", "pr_collect", "pr_rhs_batch", "pr_jac_batch",
            "pr_reaction_rate_box", "pr_box_costs", "pr_set_temperature_box", "pr_floor_species_box",
-           "pr_heat_release_box", "pr_surrogate_load", "pr_surrogate_features", "pr_surrogate_forward", "pr_mlp_gemm",
+           "pr_heat_release_box", "pr_reaction_source_box", "pr_surrogate_load", "pr_surrogate_features", "pr_surrogate_forward", "pr_mlp_gemm",
            "pr_mlp_gelu_backward", "pr_mlp_column_sum", "pr_mlp_mse", "pr_mlp_adam", "pr_mlp_to_bf16",
            "pr_get_stats", "pr_num_species", "pr_destroy", "pr_last_error"]
 
@@ -94,6 +94,8 @@ def lib():
         L.pr_floor_species_box.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), a4, a4, a4, a4,
                                            C.c_void_p]
         L.pr_heat_release_box.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), a4, a4, a4, C.c_void_p]
+        L.pr_reaction_source_box.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), a4, a4, a4, a4,
+                                             C.c_double, C.c_void_p]
         vp, i64, fp = C.c_void_p, C.c_int64, C.POINTER(C.c_float)
         L.pr_surrogate_load.argtypes = [vp, C.c_int, C.POINTER(C.c_int), fp, fp, fp]
         L.pr_surrogate_features.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
@@ -309,6 +311,22 @@ class Reactor:
             HR = torch.empty_like(T)
         self._eos("pr_heat_release_box", [_array4(T, 1), _array4(IR, self.nsp), _array4(HR, 1)], T.shape, stream)
         return HR
+
+    def reaction_source(self, rY_old, rY_new, F, dt, IR=None, stream=None):
+        """I_R = -(rY_old - rY_new) / dt - F over a box of device tensors [N, nz, ny, nx] (MKS)."""
+        import torch
+        if IR is None:
+            IR = torch.empty_like(rY_new)
+        nz, ny, nx = rY_new.shape[1:]
+        lo = (C.c_int * 3)(0, 0, 0)
+        hi = (C.c_int * 3)(nx - 1, ny - 1, nz - 1)
+        st = stream if stream is not None else torch.cuda.current_stream()
+        arrs = [_array4(x, self.nsp) for x in (rY_old, rY_new, F, IR)]
+        rc = lib().pr_reaction_source_box(self._h, lo, hi, *[C.byref(x) for x in arrs], C.c_double(float(dt)),
+                                          C.c_void_p(st.cuda_stream))
+        if rc != 0:
+            raise ReactorError(f"pr_reaction_source_box failed ({rc}): {lib().pr_last_error().decode()}")
+        return IR
 
     def load_surrogate(self, dims, params, xnorm, ynorm):
         """pr_surrogate_load: dims (layer widths), params (fp32 W0, b0, W1, b1, ... flattened),

@@ -515,3 +515,21 @@ def test_max_order_2_parity(gpu):
     assert out["nfail"] == 0
     for k in ("rY", "T", "rH", "FC"):
         _assert_bits(k, out[k], ref[k])
+
+
+def test_reaction_source_box_bits(gpu):
+    """pr_reaction_source_box (I_R formation) on a box of random MKS states: the oracle's bits."""
+    import torch
+    from oracle import pyoracle as po
+    from pelelmex_amd.reactor import Reactor
+    rng = np.random.default_rng(12)
+    shape = (21, 5, 6, 7)
+    o = rng.random(shape) * 0.2
+    n = o * (1 + 1e-3 * rng.standard_normal(shape))
+    F = rng.standard_normal(shape) * 50
+    r = Reactor.create("ReactorCvode", "drm19", KEYS)
+    r.init(2, 1)
+    dev = lambda a: torch.as_tensor(a).to(gpu).contiguous()
+    IR = r.reaction_source(dev(o), dev(n), dev(F), 3.7e-6).cpu().numpy()
+    r.close()
+    _assert_bits("I_R", IR, po.reaction_source(o, n, F, 3.7e-6))

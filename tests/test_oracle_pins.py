@@ -174,6 +174,19 @@ def test_syn53_isomer_equilibrium_constants():
     assert checked == 3 * 160
 
 
+def test_reaction_source_pins():
+    """I_R restatement: no change gives -F exactly; with F = 0 and dt a power of two the result is
+    the exact scaled difference; otherwise it agrees with (new - old) / dt - F to rounding."""
+    rng = np.random.default_rng(11)
+    o = rng.random((21, 4, 3, 5)) * 0.2
+    n = o * (1 + 1e-3 * rng.standard_normal(o.shape))
+    F = rng.standard_normal(o.shape) * 50
+    np.testing.assert_array_equal(po.reaction_source(o, o, F, 3.7e-6), -F)
+    np.testing.assert_array_equal(po.reaction_source(o, n, np.zeros_like(F), 0.5), (n - o) * 2.0)
+    ref = (n - o) / 3.7e-6 - F
+    assert np.max(np.abs(po.reaction_source(o, n, F, 3.7e-6) - ref) / np.maximum(np.abs(ref), 1e-300)) < 1e-9
+
+
 def test_falloff_limits():
     """H + O2 (+M) = HO2 (+M) in LiDryer: low pressure -> k0 [M], high pressure -> k_inf F_inf."""
     om = po.Mech("LiDryer")
