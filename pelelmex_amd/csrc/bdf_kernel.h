@@ -944,21 +944,22 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
 
 // The LU rows persist between factorizations in the group's LDS area (or, above 32 components,
 // in its global slot); registers hold them only while a factorization or a solve runs.
-// At 32 lanes (one row per lane) the rows are stored in pivot order (slot k = row pivoted at step
-// k), so that the solve finds the pivot row of step k in lane k.
+// At 32 lanes (one row per lane), and in the global slot above 32 components, the rows are
+// stored in pivot order (slot k = row pivoted at step k), so that the solve finds the pivot row of
+// step k in lane k.
 template <int N, int R, int G>
 constexpr bool lu_pivot_order() {
-  return G == 32 && Cell<N, R, G>::RPL == 1;
+  return (G == 32 && Cell<N, R, G>::RPL == 1) || Cell<N, R, G>::lu_global;
 }
 
 template <int N, int R, int G>
 __device__ __forceinline__ void store_lu(const Smem<N, R>& S, int lane, const double (&lu)[Cell<N, R, G>::RPL][N + 1],
                                          const int (&pst)[Cell<N, R, G>::RPL]) {
   constexpr int n = N + 1;
-  if constexpr (Cell<N, R, G>::lu_global) {
+  if constexpr (Cell<N, R, G>::lu_global) {  // slot pst of entry j at j * 64 + pst
+    if (pst[0] >= 0 && pst[0] < n)
 #pragma unroll
-    for (int j = 0; j < n; ++j) S.LU[j * 64 + lane] = lu[0][j];  // lanes >= n: copies, never read
-    (void)pst;
+      for (int j = 0; j < n; ++j) S.LU[j * 64 + pst[0]] = lu[0][j];
     return;
   }
 #pragma unroll
@@ -974,7 +975,7 @@ __device__ __forceinline__ void store_lu(const Smem<N, R>& S, int lane, const do
 template <int N, int R, int G>
 __device__ __forceinline__ void load_lu(const Smem<N, R>& S, int lane, double (&lu)[Cell<N, R, G>::RPL][N + 1]) {
   constexpr int n = N + 1;
-  if constexpr (Cell<N, R, G>::lu_global) {
+  if constexpr (Cell<N, R, G>::lu_global) {  // lane k gets slot k; lanes >= n are never read
 #pragma unroll
     for (int j = 0; j < n; ++j) lu[0][j] = S.LU[j * 64 + lane];
     return;
@@ -994,7 +995,7 @@ __device__ __forceinline__ void lu_solve(const Smem<N, R>& S, int lane, const do
                                          int permv, double (&b)[Cell<N, R, G>::RPL], double (&x)[Cell<N, R, G>::RPL]) {
   constexpr int RPL = Cell<N, R, G>::RPL;
   constexpr int n = N + 1;
-  if constexpr (G == 64) {
+  if constexpr (G == 64 && !lu_pivot_order<N, R, G>()) {
 #pragma unroll
     for (int k = 0; k < n; ++k) {
       const int p = __builtin_amdgcn_readlane(permv, k);
@@ -1013,12 +1014,14 @@ __device__ __forceinline__ void lu_solve(const Smem<N, R>& S, int lane, const do
   }
   if constexpr (lu_pivot_order<N, R, G>()) {
     // lane k holds the row pivoted at step k (pst = k); its right-hand side entry is b[p_k].
-    // Step k's value comes from lane k of the half through the LDS crossbar (ds_bpermute, the
+    // Step k's value comes from lane k of the group through the LDS crossbar (ds_bpermute, the
     // lane in the immediate offset): no vector instructions for the broadcast.
-    const int base = (int)(threadIdx.x & 32) * 4;
+    const int base = (G == 32) ? (int)(threadIdx.x & 32) * 4 : 0;
     const int ol = opaque(lane);  // the step masks are formed here, not hoisted and spilled
     double v = __shfl(b[0], permv, G);
-    const double rdiag = (lane < n) ? 1.0 / S.LU[lane * n + lane] : 0.0;  // the stored 1 / pivot
+    // 1 / pivot from the stored (unscaled) pivot entry: the factorization's reciprocal
+    const double rdiag =
+        (lane < n) ? 1.0 / (Cell<N, R, G>::lu_global ? S.LU[lane * 64 + lane] : S.LU[lane * n + lane]) : 0.0;
 #pragma unroll
     for (int k = 0; k < n; ++k) {
       const double yk = bperm_d(v, base + 4 * k);
