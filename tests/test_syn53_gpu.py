@@ -56,3 +56,52 @@ def test_syn53_react_parity(gpu, waves):
     _assert_bits("FC", out["FC"], ref["FC"])
     cov = box["mask"] == -1
     assert np.all(out["FC"][cov] == 0) and np.all(out["FC"][~cov] > 0)
+
+
+def test_syn53_too_much_work_and_clean_init(gpu):
+    """The 54-component path through its failure and clipping rules: cvode.max_substeps = 300
+    stops the stiffest cells with status -1 after their last accepted step, and
+    ode.clean_init_massfrac = 1 integrates from max(rhoY, 0) and adds the clipped part back -
+    status, state, rho h and FC bit for bit with the oracle."""
+    box = _sample(W.syn53_chunk(1, nx=32, ny=32, nz=32), 192, seed=8)
+    rng = np.random.default_rng(8)
+    hit = rng.random(box["rY"].shape) < 0.03
+    box["rY"] = np.where(hit, -1e-9 * box["rY"].sum(0, keepdims=True) * rng.random(box["rY"].shape), box["rY"])
+    assert np.any(box["rY"] < 0)
+    ref = _oracle(box, max_order=4, max_steps=300, clean_init_massfrac=1)
+    out = _gpu(box, dict(KEYS, **{"cvode.max_order": 4, "cvode.max_substeps": 300, "ode.clean_init_massfrac": 1}),
+               gpu)
+    failed = ref["status"] != 0
+    assert failed.sum() > 20 and (~failed & (box["mask"] != -1)).sum() > 20   # both kinds present
+    assert np.all(ref["status"][failed] == -1)
+    np.testing.assert_array_equal(out["status"], ref["status"])
+    assert out["nfail"] == int(failed.sum())
+    for k in ("rY", "T", "rH", "FC"):
+        _assert_bits(k, out[k], ref[k])
+
+
+def test_syn53_batch_matches_single_boxes(gpu):
+    """Two syn53 boxes of different sizes and dt_react in one pr_react_batch launch: box by box
+    the bits of pr_react_box."""
+    import torch
+    from pelelmex_amd.reactor import Reactor, box_tensors
+    boxes = [W.syn53_chunk(2, nx=8, ny=4, nz=4, dt=1e-6), W.syn53_chunk(3, nx=4, ny=4, nz=4, dt=4e-6)]
+    r = Reactor.create("ReactorCvode", "syn53", dict(KEYS, **{"cvode.max_order": 4}))
+    r.init(2, 1)
+    base = [box_tensors(b, gpu) for b in boxes]
+    singles = []
+    for b, t0 in zip(boxes, base):
+        t = {k: v.clone() for k, v in t0.items()}
+        assert r.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], t["mask"], b["dt"], 0.0)[0] == 0
+        singles.append(t)
+    batch = []
+    for b, t0 in zip(boxes, base):
+        t = {k: v.clone() for k, v in t0.items()}
+        t["dt"] = b["dt"]
+        batch.append(t)
+    assert r.react_batch(batch) == 0
+    torch.cuda.synchronize()
+    r.close()
+    for ts, tb in zip(singles, batch):
+        for k in ("rY", "T", "rH", "FC"):
+            _assert_bits(k, tb[k].cpu().numpy(), ts[k].cpu().numpy())
