@@ -32,6 +32,8 @@
  *   pr_set_temperature_box  PeleLM::setTemperature per box (Source/PeleLMeX_Eos.cpp:218-240,
  *                    getTfromHY Source/PeleLMeX_K.H:700-723)
  *   pr_floor_species_box    PeleLM::floorSpecies per box (Source/PeleLMeX_Utils.cpp:649-690)
+ *   pr_advance_chemistry_box  the per-box body of PeleLM::advanceChemistry(lev, dt, Forcing)
+ *                    (Source/PeleLMeX_Reactions.cpp:54-160: conversions, react, I_R)
  *   pr_reaction_source_box  I_R formation in advanceChemistry (Source/PeleLMeX_Reactions.cpp:141-160,
  *                    342-347)
  *   pr_heat_release_box     PeleLM::getHeatRelease per box (Source/PeleLMeX_Reactions.cpp:477-506,
@@ -176,6 +178,19 @@ int pr_heat_release_box(pr_handle* h, const int lo[3], const int hi[3], const pr
 int pr_reaction_source_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY_old,
                            const pr_array4* rY_new, const pr_array4* extF_rY, const pr_array4* I_R, double dt,
                            void* stream);
+
+/* One box of PeleLM::advanceChemistry(lev, dt, Forcing) in one call, on MKS views of the level
+   state: rhoY_new = rhoY_old * 1e-3, T_new = T_old, rhoH_new = rhoH_old * 10 and the forcing
+   scaled to CGS in place (Source/PeleLMeX_Reactions.cpp:68-79); pr_react_box over dt on the new
+   state (:101-108; the learned map too when loaded); the new state and the forcing scaled back
+   (:110-120); and, when I_R->p is not NULL, I_R = -(rhoY_old - rhoY_new) * (1 / dt) - F_rhoY from
+   the restored forcing (:141-160). Every product is the reference's, so the caller's forcing
+   makes the same rounding round trip. Returns the failed-cell count (>= 0) or an error code. */
+int pr_advance_chemistry_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY_old,
+                             const pr_array4* rH_old, const pr_array4* T_old, const pr_array4* rY_new,
+                             const pr_array4* rH_new, const pr_array4* T_new, const pr_array4* extF_rY,
+                             const pr_array4* extF_rH, const pr_array4* FC, const pr_iarray4* mask,
+                             const pr_array4* I_R, double dt, void* stream, int* status_out);
 /* Learned dt_react map. Network: nlayers dense layers, dims[0] = NUM_SPECIES + 2 inputs
    (log10(Y_k + 1e-20), T / 2000, log10(dt)), dims[nlayers] = NUM_SPECIES + 1 outputs (dY_k, dT),
    GELU between layers. params (host, fp32): for each layer W [dims[l] x dims[l+1]] row-major then

@@ -207,6 +207,23 @@ def reaction_source(rY_old, rY_new, F, dt):
         - np.asarray(F, dtype=np.float64)
 
 
+def advance_chemistry(mech, rY_o, rH_o, T_o, F_rY, F_rH, mask, dt, **react_kw):
+    """One box of PeleLM::advanceChemistry(lev, dt, Forcing) on MKS arrays (component-major
+    [N, ncells] / [ncells]): the CGS conversions of Source/PeleLMeX_Reactions.cpp:68-79, the
+    oracle's react, the conversions back of :110-120 (the forcing scaled to CGS and back, as the
+    reference scales it in place) and I_R of :141-160. Returns the new state, the forcing after
+    its round trip, I_R, FC and status."""
+    rYn = np.asarray(rY_o, dtype=np.float64) * 1.0e-3
+    FY = np.asarray(F_rY, dtype=np.float64) * 1.0e-3
+    rHn = np.asarray(rH_o, dtype=np.float64) * 10.0
+    FH = np.asarray(F_rH, dtype=np.float64) * 10.0
+    out = mech.react(rYn, FY, np.array(T_o, dtype=np.float64), rHn, FH, mask, dt, **react_kw)
+    rYn = out["rY"] * 1.0e3
+    FY = FY * 1.0e3
+    return dict(rY=rYn, rH=out["rH"] * 0.1, T=out["T"], FY=FY, FH=FH * 0.1, FC=out["FC"], status=out["status"],
+                IR=reaction_source(rY_o, rYn, FY, dt))
+
+
 def exp(x):
     return lib().orc_exp(float(x))
 

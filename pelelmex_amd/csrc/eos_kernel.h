@@ -162,4 +162,50 @@ __global__ void __launch_bounds__(256) ir_kernel(IrParams P) {
   }
 }
 
+// The unit conversions around react in advanceChemistry (Source/PeleLMeX_Reactions.cpp:68-79
+// before, :110-120 after), one thread per cell, MKS views:
+//  before: rhoY_new = rhoY_old * 1e-3, F_rhoY *= 1e-3, T_new = T_old, rhoH_new = rhoH_old * 10,
+//          F_rhoH *= 10 (the forcing is scaled in place, as the reference scales it);
+//  after:  rhoY_new *= 1e3, F_rhoY *= 1e3, rhoH_new *= 0.1, F_rhoH *= 0.1, then
+//          I_R = -(rhoY_old - rhoY_new) * dt_inv - F_rhoY (:141-160) from the restored forcing.
+struct ChemBoxParams {
+  int lo0, lo1, lo2, nx, ny;
+  long ncells;
+  int N;
+  Arr rYo, rHo, To, rYn, rHn, Tn, FY, FH, IR;
+  double dt_inv;
+};
+
+template <bool AFTER>
+__global__ void __launch_bounds__(256) chem_units_kernel(ChemBoxParams P) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < P.ncells; c += stride) {
+    const unsigned u = (unsigned)c;
+    const int i = P.lo0 + (int)(u % (unsigned)P.nx);
+    const int j = P.lo1 + (int)((u / (unsigned)P.nx) % (unsigned)P.ny);
+    const int k = P.lo2 + (int)(u / ((unsigned)P.nx * (unsigned)P.ny));
+    const long oYo = aoff(P.rYo, i, j, k), oYn = aoff(P.rYn, i, j, k), oF = aoff(P.FY, i, j, k);
+    if (!AFTER) {
+      for (int n = 0; n < P.N; ++n) {
+        P.rYn.p[oYn + n * P.rYn.ns] = P.rYo.p[oYo + n * P.rYo.ns] * 1.0e-3;
+        P.FY.p[oF + n * P.FY.ns] *= 1.0e-3;
+      }
+      P.Tn.p[aoff(P.Tn, i, j, k)] = P.To.p[aoff(P.To, i, j, k)];
+      P.rHn.p[aoff(P.rHn, i, j, k)] = P.rHo.p[aoff(P.rHo, i, j, k)] * 10.0;
+      P.FH.p[aoff(P.FH, i, j, k)] *= 10.0;
+    } else {
+      const long oI = aoff(P.IR, i, j, k);
+      for (int n = 0; n < P.N; ++n) {
+        const double w = P.rYn.p[oYn + n * P.rYn.ns] * 1.0e3;
+        const double f = P.FY.p[oF + n * P.FY.ns] * 1.0e3;
+        P.rYn.p[oYn + n * P.rYn.ns] = w;
+        P.FY.p[oF + n * P.FY.ns] = f;
+        if (P.IR.p) P.IR.p[oI + n * P.IR.ns] = -(P.rYo.p[oYo + n * P.rYo.ns] - w) * P.dt_inv - f;
+      }
+      P.rHn.p[aoff(P.rHn, i, j, k)] *= 0.1;
+      P.FH.p[aoff(P.FH, i, j, k)] *= 0.1;
+    }
+  }
+}
+
 }  // namespace prk

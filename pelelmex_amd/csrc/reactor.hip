@@ -1560,6 +1560,60 @@ int pr_reaction_source_box(pr_handle* h, const int lo[3], const int hi[3], const
   return 0;
 }
 
+int pr_advance_chemistry_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY_old,
+                             const pr_array4* rH_old, const pr_array4* T_old, const pr_array4* rY_new,
+                             const pr_array4* rH_new, const pr_array4* T_new, const pr_array4* extF_rY,
+                             const pr_array4* extF_rH, const pr_array4* FC, const pr_iarray4* mask,
+                             const pr_array4* I_R, double dt, void* stream, int* status_out) {
+  if (!h || !lo || !hi) return fail(PR_ERR_ARG, "NULL argument");
+  const int N = h->mech->nsp;
+  if (!arr_ok(rY_old, N) || !arr_ok(rH_old, 1) || !arr_ok(T_old, 1) || !arr_ok(rY_new, N) ||
+      !arr_ok(rH_new, 1) || !arr_ok(T_new, 1) || !arr_ok(extF_rY, N) || !arr_ok(extF_rH, 1) || !arr_ok(FC, 1))
+    return fail(PR_ERR_ARG, "missing array");
+  for (const pr_array4* a : {rY_old, rY_new, extF_rY})
+    if (a->ncomp && a->ncomp < N) return fail(PR_ERR_ARG, "species arrays need NUM_SPECIES components");
+  if (I_R && I_R->p && I_R->ncomp && I_R->ncomp < N) return fail(PR_ERR_ARG, "I_R needs NUM_SPECIES components");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(PR_ERR_ARG, "dt must be positive and finite");
+  const long nx = (long)hi[0] - lo[0] + 1, ny = (long)hi[1] - lo[1] + 1, nz = (long)hi[2] - lo[2] + 1;
+  if (nx <= 0 || ny <= 0 || nz <= 0) return fail(PR_ERR_ARG, "empty box");
+  const long ncells = nx * ny * nz;
+  if (ncells > 0x7fffffffL) return fail(PR_ERR_ARG, "box too large");
+  HIPCHK(hipSetDevice(h->device));
+  prk::ChemBoxParams P{};
+  P.lo0 = lo[0];
+  P.lo1 = lo[1];
+  P.lo2 = lo[2];
+  P.nx = (int)nx;
+  P.ny = (int)ny;
+  P.ncells = ncells;
+  P.N = N;
+  P.rYo = to_arr(rY_old);
+  P.rHo = to_arr(rH_old);
+  P.To = to_arr(T_old);
+  P.rYn = to_arr(rY_new);
+  P.rHn = to_arr(rH_new);
+  P.Tn = to_arr(T_new);
+  P.FY = to_arr(extF_rY);
+  P.FH = to_arr(extF_rH);
+  if (I_R && I_R->p) P.IR = to_arr(I_R);
+  P.dt_inv = 1.0 / dt;
+  int dev_cus = 0;
+  HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, h->device));
+  const long want = (ncells + 255) / 256, cap = 16L * dev_cus;
+  const unsigned grid = (unsigned)(want < cap ? want : cap);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(prk::chem_units_kernel<false>, dim3(grid), dim3(256), 0, st, P);
+  HIPCHK(hipGetLastError());
+  double dt_react = dt, time = 0.0;
+  const int nfail = pr_react_box(h, lo, hi, rY_new, extF_rY, T_new, rH_new, extF_rH, FC, mask, &dt_react, &time,
+                                 stream, status_out);
+  if (nfail < 0) return nfail;
+  hipLaunchKernelGGL(prk::chem_units_kernel<true>, dim3(grid), dim3(256), 0, st, P);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  return nfail;
+}
+
 int pr_surrogate_load(pr_handle* h, int nlayers, const int* dims, const float* params, const float* xnorm,
                       const float* ynorm) {
   if (!h || !dims || !params || !xnorm || !ynorm || nlayers < 1 || nlayers > 64) return fail(PR_ERR_ARG, "bad argument");

@@ -533,3 +533,42 @@ def test_reaction_source_box_bits(gpu):
     IR = r.reaction_source(dev(o), dev(n), dev(F), 3.7e-6).cpu().numpy()
     r.close()
     _assert_bits("I_R", IR, po.reaction_source(o, n, F, 3.7e-6))
+
+
+def test_advance_chemistry_box_bits(gpu):
+    """pr_advance_chemistry_box (one box of advanceChemistry on MKS views: conversions, react,
+    conversions back, I_R) against the same sequence around the oracle's react: new state,
+    forcing after its round trip, I_R and FC bit for bit, covered cells untouched by react."""
+    import torch
+    from oracle import pyoracle as po
+    from pelelmex_amd.reactor import Reactor
+    b = W.ch4_air_flame_box(nx=16, ny=4, nz=2, seed=61, dt=3e-6)
+    b["mask"][::6] = -1
+    N, n = b["rY"].shape
+    # the workload's CGS state as the MKS level data PeleLMeX holds
+    rY_o, rH_o, T_o = b["rY"] * 1.0e3, b["rH"] * 0.1, b["T"].copy()
+    F_rY, F_rH = b["rYsrc"] * 1.0e3, b["rHsrc"] * 0.1
+    ref = po.advance_chemistry(po.Mech("drm19"), rY_o, rH_o, T_o, F_rY, F_rH, b["mask"], b["dt"], max_order=4)
+    nx, ny, nz = b["dims"]
+    dev = lambda a, c: torch.as_tensor(np.ascontiguousarray(a)).reshape((c, nz, ny, nx) if c else (nz, ny, nx)).to(gpu)
+    old = {"rY": dev(rY_o, N), "rH": dev(rH_o, 0), "T": dev(T_o, 0)}
+    new = {"rY": torch.zeros_like(old["rY"]), "rH": torch.zeros_like(old["rH"]), "T": torch.zeros_like(old["T"])}
+    FY, FH = dev(F_rY, N), dev(F_rH, 0)
+    FC = torch.zeros_like(old["T"])
+    IR = torch.zeros_like(old["rY"])
+    mask = torch.as_tensor(b["mask"]).reshape(nz, ny, nx).to(gpu)
+    r = Reactor.create("ReactorCvode", "drm19", dict(KEYS, **{"cvode.max_order": 4}))
+    r.init(2, 1)
+    assert r.advance_chemistry(old, new, FY, FH, FC, mask, b["dt"], IR=IR) == 0
+    torch.cuda.synchronize()
+    r.close()
+    host = lambda t, c: t.cpu().numpy().reshape((c, n) if c else (n,))
+    _assert_bits("rY", host(new["rY"], N), ref["rY"])
+    _assert_bits("rH", host(new["rH"], 0), ref["rH"])
+    _assert_bits("T", host(new["T"], 0), ref["T"])
+    _assert_bits("F_rY", host(FY, N), ref["FY"])
+    _assert_bits("F_rH", host(FH, 0), ref["FH"])
+    _assert_bits("I_R", host(IR, N), ref["IR"])
+    _assert_bits("FC", host(FC, 0), ref["FC"])
+    cov = b["mask"] == -1
+    assert np.all(ref["FC"][cov] == 0) and np.all(ref["FC"][~cov] > 0)
