@@ -93,6 +93,24 @@ class ReactorHipBDF {
     return rc;
   }
 
+  // The per-box body of PeleLM::advanceChemistry(lev, dt, Forcing)
+  // (Source/PeleLMeX_Reactions.cpp:54-160) on MKS arrays: new = old in CGS, react, back to MKS,
+  // with the forcing scaled in place as the reference scales it, and I_R when I_R.p != nullptr.
+  template <class Box, class A4, class I4>
+  int advance_chemistry(const Box& box, A4 const& rhoY_o, A4 const& rhoH_o, A4 const& temp_o, A4 const& rhoY_n,
+                        A4 const& rhoH_n, A4 const& temp_n, A4 const& extF_rhoY, A4 const& extF_rhoH, A4 const& fcl,
+                        I4 const& mask, A4 const& I_R, double dt, void* stream = nullptr) {
+    const int lo[3] = {box.smallEnd(0), box.smallEnd(1), box.smallEnd(2)};
+    const int hi[3] = {box.bigEnd(0), box.bigEnd(1), box.bigEnd(2)};
+    pr_array4 a[10] = {to_pr(rhoY_o), to_pr(rhoH_o), to_pr(temp_o), to_pr(rhoY_n),    to_pr(rhoH_n),
+                       to_pr(temp_n), to_pr(extF_rhoY), to_pr(extF_rhoH), to_pr(fcl), to_pr(I_R)};
+    pr_iarray4 m = to_pr_mask(mask);
+    int rc = pr_advance_chemistry_box(h_, lo, hi, &a[0], &a[1], &a[2], &a[3], &a[4], &a[5], &a[6], &a[7], &a[8], &m,
+                                      &a[9], dt, stream, nullptr);
+    if (rc < 0) throw ReactorError(std::string("pr_advance_chemistry_box: ") + pr_last_error());
+    return rc;
+  }
+
   void close() {
     if (h_) pr_destroy(h_);
     h_ = nullptr;

@@ -118,6 +118,10 @@ int main() {
 template int pele_hip::ReactorHipBDF::react<Box, Array4<double>, Array4<int>>(
     const Box&, Array4<double> const&, Array4<double> const&, Array4<double> const&, Array4<double> const&,
     Array4<double> const&, Array4<double> const&, Array4<int> const&, double&, double&, void*);
+template int pele_hip::ReactorHipBDF::advance_chemistry<Box, Array4<double>, Array4<int>>(
+    const Box&, Array4<double> const&, Array4<double> const&, Array4<double> const&, Array4<double> const&,
+    Array4<double> const&, Array4<double> const&, Array4<double> const&, Array4<double> const&,
+    Array4<double> const&, Array4<int> const&, Array4<double> const&, double, void*);
 ''')
     import subprocess
     exe = tmp_path / "adapter"
