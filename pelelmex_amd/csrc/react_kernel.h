@@ -970,26 +970,44 @@ __device__ __forceinline__ int cost_bucket(double fc) {
   return COST_BUCKETS - 1 - key;
 }
 
-// The cells to order: list[0 .. min(*nlist, nmax)) when list is set, else the whole box.
-__device__ __forceinline__ long cost_count(const BoxDesc& B, const int* list, const unsigned int* nlist, long nmax) {
-  if (!list) return B.ncells;
+// The cells to order: list[0 .. min(*nlist, nmax)) when list is set, else queue entries 0 .. nmax.
+__device__ __forceinline__ long cost_count(const int* list, const unsigned int* nlist, long nmax) {
+  if (!list) return nmax;
   const long n = (long)*nlist;
   return n < nmax ? n : nmax;
 }
 
-__global__ void __launch_bounds__(256) cost_hist_kernel(BoxDesc B, const int* list, const unsigned int* nlist,
-                                                        long nmax, unsigned int* hist) {
+// FC of queue entry c of a launch's box table (box-local index when there is one box).
+__device__ __forceinline__ double cost_fc(const BoxDesc* boxes, const long* box_start, int nbox, long c) {
+  int b = 0;
+  if (nbox > 1) {
+    int lo = 0, hi = nbox;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (box_start[mid] <= c) lo = mid;
+      else hi = mid;
+    }
+    b = lo;
+    c -= box_start[b];
+  }
+  const BoxDesc& B = boxes[b];
+  const unsigned u = (unsigned)c;
+  const int i = B.lo0 + (int)(u % (unsigned)B.nx);
+  const int j = B.lo1 + (int)((u / (unsigned)B.nx) % (unsigned)B.ny);
+  const int k = B.lo2 + (int)(u / ((unsigned)B.nx * (unsigned)B.ny));
+  return B.FC.p[aoff(B.FC, i, j, k)];
+}
+
+__global__ void __launch_bounds__(256) cost_hist_kernel(const BoxDesc* boxes, const long* box_start, int nbox,
+                                                        const int* list, const unsigned int* nlist, long nmax,
+                                                        unsigned int* hist) {
   __shared__ unsigned int h[COST_BUCKETS];
   for (int b = threadIdx.x; b < COST_BUCKETS; b += blockDim.x) h[b] = 0u;
   __syncthreads();
-  const long n = cost_count(B, list, nlist, nmax);
+  const long n = cost_count(list, nlist, nmax);
   for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x) {
     const long c = list ? (long)list[x] : x;
-    const unsigned u = (unsigned)c;
-    const int i = B.lo0 + (int)(u % (unsigned)B.nx);
-    const int j = B.lo1 + (int)((u / (unsigned)B.nx) % (unsigned)B.ny);
-    const int k = B.lo2 + (int)(u / ((unsigned)B.nx * (unsigned)B.ny));
-    atomicAdd(&h[cost_bucket(B.FC.p[aoff(B.FC, i, j, k)])], 1u);  // block-private counts in LDS
+    atomicAdd(&h[cost_bucket(cost_fc(boxes, box_start, nbox, c))], 1u);  // block-private counts in LDS
   }
   __syncthreads();
   for (int b = threadIdx.x; b < COST_BUCKETS; b += blockDim.x)
@@ -1009,23 +1027,20 @@ __global__ void cost_scan_kernel(unsigned int* hist) {
 // Each block counts its cells per bucket in LDS, reserves one range per bucket from the global
 // cursors, and places its cells inside its ranges (one grid-stride round per launch: the grid
 // covers every cell, so each thread handles at most one).
-__global__ void __launch_bounds__(256) cost_scatter_kernel(BoxDesc B, const int* list, const unsigned int* nlist,
-                                                           long nmax, unsigned int* hist, int* order) {
+__global__ void __launch_bounds__(256) cost_scatter_kernel(const BoxDesc* boxes, const long* box_start, int nbox,
+                                                           const int* list, const unsigned int* nlist, long nmax,
+                                                           unsigned int* hist, int* order) {
   __shared__ unsigned int cnt[COST_BUCKETS], base[COST_BUCKETS];
   for (int b = threadIdx.x; b < COST_BUCKETS; b += blockDim.x) cnt[b] = 0u;
   __syncthreads();
-  const long n = cost_count(B, list, nlist, nmax);
+  const long n = cost_count(list, nlist, nmax);
   const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long c = -1;
   int b = 0;
   unsigned int r = 0;
   if (x < n) {
     c = list ? (long)list[x] : x;
-    const unsigned u = (unsigned)c;
-    const int i = B.lo0 + (int)(u % (unsigned)B.nx);
-    const int j = B.lo1 + (int)((u / (unsigned)B.nx) % (unsigned)B.ny);
-    const int k = B.lo2 + (int)(u / ((unsigned)B.nx * (unsigned)B.ny));
-    b = cost_bucket(B.FC.p[aoff(B.FC, i, j, k)]);
+    b = cost_bucket(cost_fc(boxes, box_start, nbox, c));
     r = atomicAdd(&cnt[b], 1u);
   }
   __syncthreads();

@@ -787,6 +787,33 @@ static int autotune(pr_handle* h, const prk::KParams& Pbase, const prk::BoxDesc&
   return 0;
 }
 
+// hipbdf.cost_order: the launch's queue entries (all, or the listed ones) handed out by
+// decreasing input FC, the previous advance's cost; P.cell_list becomes the ordered list (the
+// listed count P.ncells_dev is unchanged). No memset and no allocation once sized for ncells.
+static int cost_order_launch(pr_handle* h, prk::KParams& P, long ncells, hipStream_t st) {
+  if (h->order_cap < ncells) {
+    if (h->d_order) (void)hipFree(h->d_order);
+    h->d_order = nullptr;
+    h->order_cap = 0;
+    HIPCHK(hipMalloc(&h->d_order, sizeof(int) * (size_t)ncells));
+    h->order_cap = ncells;
+  }
+  if (!h->d_hist) HIPCHK(hipMalloc(&h->d_hist, sizeof(unsigned int) * 2 * prk::COST_BUCKETS));
+  hipLaunchKernelGGL(prk::zero_u32, dim3(1), dim3(256), 0, st, h->d_hist, 2 * prk::COST_BUCKETS);
+  const int* list = P.cell_list;
+  const unsigned int* nlist = P.ncells_dev;
+  const unsigned hgrid = (unsigned)((ncells + 255) / 256 < 1024 ? (ncells + 255) / 256 : 1024);
+  const unsigned sgrid = (unsigned)((ncells + 255) / 256);  // one entry per thread
+  hipLaunchKernelGGL(prk::cost_hist_kernel, dim3(hgrid), dim3(256), 0, st, P.boxes, P.box_start, P.nbox, list, nlist,
+                     ncells, h->d_hist);
+  hipLaunchKernelGGL(prk::cost_scan_kernel, dim3(1), dim3(64), 0, st, h->d_hist);
+  hipLaunchKernelGGL(prk::cost_scatter_kernel, dim3(sgrid), dim3(256), 0, st, P.boxes, P.box_start, P.nbox, list,
+                     nlist, ncells, h->d_hist, h->d_order);
+  HIPCHK(hipGetLastError());
+  P.cell_list = h->d_order;
+  return 0;
+}
+
 // Blocks for a grid-stride launch of the surrogate's per-cell kernels over ncells cells.
 static int surr_blocks(pr_handle* h, long ncells, int* block, size_t* shmem, int* gblocks) {
   const int G = h->var->G;
@@ -1084,27 +1111,8 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   P.box_start = dstart;
   P.nbox = 1;
   if (h->keys.cost_order) {
-    // hipbdf.cost_order: the cells (all, or the learned map's rejects) handed out by decreasing
-    // input FC, the previous advance's cost
-    if (h->order_cap < ncells) {
-      if (h->d_order) (void)hipFree(h->d_order);
-      h->d_order = nullptr;
-      h->order_cap = 0;
-      HIPCHK(hipMalloc(&h->d_order, sizeof(int) * (size_t)ncells));
-      h->order_cap = ncells;
-    }
-    if (!h->d_hist) HIPCHK(hipMalloc(&h->d_hist, sizeof(unsigned int) * 2 * prk::COST_BUCKETS));
-    HIPCHK(hipMemsetAsync(h->d_hist, 0, sizeof(unsigned int) * 2 * prk::COST_BUCKETS, st));
-    const int* list = P.cell_list;
-    const unsigned int* nlist = P.ncells_dev;
-    const unsigned hgrid = (unsigned)((ncells + 255) / 256 < 1024 ? (ncells + 255) / 256 : 1024);
-    const unsigned sgrid = (unsigned)((ncells + 255) / 256);  // one cell per thread
-    hipLaunchKernelGGL(prk::cost_hist_kernel, dim3(hgrid), dim3(256), 0, st, P.one, list, nlist, ncells, h->d_hist);
-    hipLaunchKernelGGL(prk::cost_scan_kernel, dim3(1), dim3(64), 0, st, h->d_hist);
-    hipLaunchKernelGGL(prk::cost_scatter_kernel, dim3(sgrid), dim3(256), 0, st, P.one, list, nlist, ncells, h->d_hist,
-                       h->d_order);
-    HIPCHK(hipGetLastError());
-    P.cell_list = h->d_order;  // P.ncells_dev (the reject count) is unchanged
+    int rc = cost_order_launch(h, P, ncells, st);
+    if (rc) return rc;
   }
   {
     int rc = bdf_launch(h, P, ncells, st);
@@ -1258,6 +1266,10 @@ int pr_react_batch_enqueue(pr_handle* h, int nbox, const pr_box* boxes, void* st
     P.boxes = dboxes;
     P.box_start = dstart;
     P.nbox = nbox;
+  }
+  if (h->keys.cost_order) {  // cells of every box by decreasing input FC
+    int rc = cost_order_launch(h, P, total, st);
+    if (rc) return rc;
   }
   {
     int rc = bdf_launch(h, P, total, st);
