@@ -477,6 +477,8 @@ struct Smem {
   double* q;     // R rates of progress
   int* perm;     // N+1 pivot rows
   double* LU;    // (N+1)^2 LU factors, row-major; rows live in registers only during factor / solve
+  double* Lp;    // N+1 doubles of LDS: the pivot-row broadcast of a 32-lane factorization whose
+                 // factors are kept in the global slot
   static constexpr int n = N + 1;
   // per-group LDS (doubles) and per-group global scratch slot (doubles)
   static constexpr int scratch_doubles = 5 * N + R + (n + 1) / 2 + 1;
@@ -488,17 +490,19 @@ struct Cell {
   static constexpr int n = N + 1;
   static constexpr int RPL = (n + G - 1) / G;
   static constexpr int RXL = (R + G - 1) / G;
-  // Above 32 components (one row per lane of a 64-lane group) the group's LDS holds no n x n
-  // area: the LU factors live in the group's global slot, stored entry by entry with the 64
-  // lanes' rows side by side (entry j of lane l's row at j * 64 + l, so every access coalesces),
-  // and the Jacobian is assembled in the slot itself.
-  static constexpr bool lu_global = (G == 64 && n > 32);
+  // One row per lane at 32 lanes, and above 32 components at 64 lanes: the group's LDS holds no
+  // n x n area. The LU factors live in the group's global slot, stored entry by entry with the n
+  // rows side by side (entry j of slot s at j * n + s, so every access coalesces; lanes past n
+  // read the last slot's address, which adds no memory traffic), and the Jacobian is assembled
+  // in the slot itself. A 32-lane factorization broadcasts its pivot rows through n doubles of
+  // LDS.
+  static constexpr bool lu_global = (G == 64 && n > 32) || (G == 32 && RPL == 1);
   // Otherwise, with one cell per wavefront the saved Jacobian and its records sit in LDS after
   // the group's scratch (few groups per block); narrower groups keep them in a global slot.
   static constexpr bool js_lds = (G == 64 && !lu_global);
-  static constexpr int lds_doubles =
-      Smem<N, R>::scratch_doubles + (lu_global ? 0 : n * n) + (js_lds ? Smem<N, R>::slot_doubles : 0);
-  static constexpr int gslot_doubles = js_lds ? 0 : Smem<N, R>::slot_doubles + (lu_global ? 64 * n : 0);
+  static constexpr int lds_doubles = Smem<N, R>::scratch_doubles + (lu_global ? (G == 32 ? n : 0) : n * n) +
+                                     (js_lds ? Smem<N, R>::slot_doubles : 0);
+  static constexpr int gslot_doubles = js_lds ? 0 : Smem<N, R>::slot_doubles + (lu_global ? n * n : 0);
 };
 
 // ---------------------------------------------------------------- right-hand side
@@ -861,19 +865,22 @@ __device__ __forceinline__ bool lu_factor(const Smem<N, R>& S, int lane, double 
       const int owner = upper_half() ? o1 - 32 : o0;
       if (lane == owner) pst[0] = k;
       if (ol == k) permv = owner;
-      // the pivot row (final row k of U) goes through the group's LU area in LDS, which holds
-      // nothing during the factorization (the factors are stored after it)
+      // the pivot row (final row k of U) goes through LDS: the group's LU area, which holds
+      // nothing during the factorization (the factors are stored after it), or the pivot-row
+      // buffer when the factors are kept in the global slot (a wavefront's LDS operations complete
+      // in order, so step k + 1's write follows step k's reads)
+      double* const prow = Cell<N, R, G>::lu_global ? S.Lp : S.LU + k * n;
       if (lane == owner)
 #pragma unroll
-        for (int j = k; j < n; ++j) S.LU[k * n + j] = lu[0][j];
+        for (int j = k; j < n; ++j) prow[j] = lu[0][j];
       group_sync();
-      const double inv = 1.0 / S.LU[k * n + k];
+      const double inv = 1.0 / prow[k];
       if (lane == owner) rinv[0] = inv;
       if (pst[0] > k) {  // rows still to be pivoted: one masked region, no per-element selects
         lu[0][k] *= inv;
         const double m = lu[0][k];
 #pragma unroll
-        for (int j = k + 1; j < n; ++j) lu[0][j] = fma(-m, S.LU[k * n + j], lu[0][j]);
+        for (int j = k + 1; j < n; ++j) lu[0][j] = fma(-m, prow[j], lu[0][j]);
       }
     }
     group_sync();
@@ -956,10 +963,10 @@ template <int N, int R, int G>
 __device__ __forceinline__ void store_lu(const Smem<N, R>& S, int lane, const double (&lu)[Cell<N, R, G>::RPL][N + 1],
                                          const int (&pst)[Cell<N, R, G>::RPL]) {
   constexpr int n = N + 1;
-  if constexpr (Cell<N, R, G>::lu_global) {  // slot pst of entry j at j * 64 + pst
+  if constexpr (Cell<N, R, G>::lu_global) {  // slot pst of entry j at j * n + pst
     if (pst[0] >= 0 && pst[0] < n)
 #pragma unroll
-      for (int j = 0; j < n; ++j) S.LU[j * 64 + pst[0]] = lu[0][j];
+      for (int j = 0; j < n; ++j) S.LU[j * n + pst[0]] = lu[0][j];
     return;
   }
 #pragma unroll
@@ -975,9 +982,10 @@ __device__ __forceinline__ void store_lu(const Smem<N, R>& S, int lane, const do
 template <int N, int R, int G>
 __device__ __forceinline__ void load_lu(const Smem<N, R>& S, int lane, double (&lu)[Cell<N, R, G>::RPL][N + 1]) {
   constexpr int n = N + 1;
-  if constexpr (Cell<N, R, G>::lu_global) {  // lane k gets slot k; lanes >= n are never read
+  if constexpr (Cell<N, R, G>::lu_global) {  // lane k gets slot k; lanes >= n (a copy of slot n - 1) are never read
+    const int sl = lane < n ? lane : n - 1;
 #pragma unroll
-    for (int j = 0; j < n; ++j) lu[0][j] = S.LU[j * 64 + lane];
+    for (int j = 0; j < n; ++j) lu[0][j] = S.LU[j * n + sl];
     return;
   }
 #pragma unroll
@@ -1021,7 +1029,7 @@ __device__ __forceinline__ void lu_solve(const Smem<N, R>& S, int lane, const do
     double v = __shfl(b[0], permv, G);
     // 1 / pivot from the stored (unscaled) pivot entry: the factorization's reciprocal
     const double rdiag =
-        (lane < n) ? 1.0 / (Cell<N, R, G>::lu_global ? S.LU[lane * 64 + lane] : S.LU[lane * n + lane]) : 0.0;
+        (lane < n) ? 1.0 / S.LU[lane * n + lane] : 0.0;  // the diagonal entry in either layout
 #pragma unroll
     for (int k = 0; k < n; ++k) {
       const double yk = bperm_d(v, base + 4 * k);

@@ -63,6 +63,7 @@ __device__ __forceinline__ double* setup_views(Smem<N, R>& S, unsigned char* sme
   S.q = S.sCpm + N;
   S.perm = reinterpret_cast<int*>(S.q + R);
   S.LU = Cell<N, R, G>::lu_global ? nullptr : S.q + R + (n + 1) / 2 + 1;
+  S.Lp = (Cell<N, R, G>::lu_global && G == 32) ? S.q + R + (n + 1) / 2 + 1 : nullptr;
   if constexpr (Cell<N, R, G>::js_lds) {
     S.Js = S.LU + n * n;
     S.jrec = S.Js + n * n;
@@ -70,7 +71,7 @@ __device__ __forceinline__ double* setup_views(Smem<N, R>& S, unsigned char* sme
     const long slot = (long)blockIdx.x * (blockDim.x / G) + grp;
     S.Js = gscr + slot * Cell<N, R, G>::gslot_doubles;
     S.jrec = S.Js + n * n;
-    if constexpr (Cell<N, R, G>::lu_global) S.LU = S.jrec + 4 * R;  // 64 n doubles after the records
+    if constexpr (Cell<N, R, G>::lu_global) S.LU = S.jrec + 4 * R;  // n n doubles after the records
   } else {
     S.Js = nullptr;  // kernels without a Jacobian
     S.jrec = nullptr;

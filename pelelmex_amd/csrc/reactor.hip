@@ -149,16 +149,17 @@ struct Variant {
   {NAME, G, W, &launch_t<NS, NR, G, W>, &eval_t<NS, NR, G>, &occ_t<NS, NR, G, W>, scratch_bytes<NS, NR, G>(), \
    prk::Cell<NS, NR, G>::gslot_doubles * 8, &rate_t<NS, NR, G>, &trust_t<NS, NR, G>, &feat_t<NS>, &prep_t<NS, NR, G>, &eos_t<NS, NR, G>}
 
-// drm19 at 32 lanes runs 3 waves per SIMD (24 cells in a 768-thread block, 159 KB of LDS per
-// CU): some registers spill, yet it is 18% faster than 2 waves with none (the integrator is
-// latency bound). At 64 lanes the group's LDS does not fit 3 waves; at 16 lanes 4 cells share
-// a wave and registers already spill at 2. LiDryer at 16 lanes fits 3 waves without spills
-// (21% faster than 2 on a 32^3 box); at 8 lanes 4 waves spill and are slower. A 54-component
-// state (syn53) keeps its LU factors in its global slot: 229 VGPRs at 2 waves (no spills), 52
-// spilled at 3.
+// drm19 at 32 lanes keeps its LU factors in the global slot (the group's LDS then holds 2 KB
+// instead of 6 KB), which lets 4 waves per SIMD be resident: 32 cells in a 1024-thread block, 44
+// VGPRs spilled, 7 % faster than 3 waves with the factors in LDS (the integrator is latency
+// bound). At 64 lanes the group's LDS does not fit 3 waves; at 16 lanes 4 cells share a wave and
+// registers already spill at 2. LiDryer at 16 lanes fits 3 waves without spills (21% faster than
+// 2 on a 32^3 box); at 8 lanes 4 waves spill and are slower. A 54-component state (syn53) keeps
+// its LU factors in its global slot too: 223 VGPRs at 2 waves (no spills), 45 spilled at 3.
 const Variant kVariants[] = {
     PR_VARIANT("LiDryer", PR_LIDRYER_NSP, PR_LIDRYER_NR, 16, 3),
     PR_VARIANT("LiDryer", PR_LIDRYER_NSP, PR_LIDRYER_NR, 8, 2),
+    PR_VARIANT("drm19", PR_DRM19_NSP, PR_DRM19_NR, 32, 4),
     PR_VARIANT("drm19", PR_DRM19_NSP, PR_DRM19_NR, 32, 3),
     PR_VARIANT("drm19", PR_DRM19_NSP, PR_DRM19_NR, 16, 2),
     PR_VARIANT("drm19", PR_DRM19_NSP, PR_DRM19_NR, 64, 2),
@@ -329,8 +330,8 @@ int parse_keys(const char* text, Keys& K) {
       if (!num(d) || (d != 0 && d != 1)) return fail(PR_ERR_KEY, "hipbdf.autotune must be 0 or 1");
       K.autotune = (int)d;
     } else if (k == "hipbdf.block") {
-      if (!num(d) || ((int)d % 64) != 0 || d < 64 || d > 768)
-        return fail(PR_ERR_KEY, "hipbdf.block must be a multiple of 64 from 64 to 768");
+      if (!num(d) || ((int)d % 64) != 0 || d < 64 || d > 1024)
+        return fail(PR_ERR_KEY, "hipbdf.block must be a multiple of 64 from 64 to 1024");
       K.block = (int)d;
     } else if (k == "surrogate.enable") {
       if (!num(d)) return fail(PR_ERR_KEY, "bad surrogate.enable");

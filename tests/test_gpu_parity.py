@@ -91,17 +91,17 @@ def test_h2_air_parity(gpu, lanes):
     assert np.all(out["FC"] > 0)
 
 
-@pytest.mark.parametrize("lanes", [32, 16, 64])
-def test_ch4_flame_parity(gpu, lanes):
+@pytest.mark.parametrize("lanes,waves", [(32, 0), (32, 3), (16, 0), (64, 0)])
+def test_ch4_flame_parity(gpu, lanes, waves):
     full = W.ch4_air_flame_box()
     # every slab type: one full i-row plus a random sample
     idx = np.union1d(np.arange(64), np.random.default_rng(2).choice(full["T"].size, 448, replace=False))
     box = W.subsample(full, idx)
     ref = _oracle(box, max_order=5)
-    out = _gpu(box, dict(KEYS, **{"hipbdf.lanes_per_cell": lanes}), gpu)
+    out = _gpu(box, dict(KEYS, **{"hipbdf.lanes_per_cell": lanes, "hipbdf.waves_per_simd": waves}), gpu)
     assert out["nfail"] == 0 and np.all(out["status"] == 0)
     dT, dY = _compare(out, ref, box["mask"])
-    print(f"drm19 G={lanes}: max dT/T {dT:.2e} max dY {dY:.2e} stats {out['stats']}")
+    print(f"drm19 G={lanes} W={out['stats']['waves_per_simd']}: max dT/T {dT:.2e} max dY {dY:.2e} stats {out['stats']}")
     assert dT <= T_TOL and dY <= Y_TOL
     _assert_bits("rY", out["rY"], ref["rY"])
     _assert_bits("T", out["T"], ref["T"])
@@ -305,8 +305,8 @@ def test_autotune_picks_a_variant_with_identical_results(gpu):
 
 
 def test_block_size_changes_nothing_but_speed(gpu):
-    """The 32-lane drm19 kernel runs 3 waves per SIMD in 768-thread blocks by default;
-    hipbdf.block = 256 and 128 give the same bits. A block above a variant's limit (768 threads
+    """The 32-lane drm19 kernel runs 4 waves per SIMD in 1024-thread blocks by default;
+    hipbdf.block = 256 and 128 give the same bits. A block above a variant's limit (512 threads
     at 64 lanes, compiled for 2 waves per SIMD) is refused before anything is launched."""
     from pelelmex_amd.reactor import ReactorError
     box = W.ch4_air_flame_box(nx=32, ny=8, nz=8, seed=37)
