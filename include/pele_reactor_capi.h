@@ -89,8 +89,12 @@ typedef struct {
 #define PR_ERR_KEY (-4)
 
 /* integrator: "ReactorCvode" (drop-in name) or "ReactorHipBDF"; mechanism: "LiDryer",
-   "drm19"; reactor_type: 2 (constant pressure, rho h) is the one PeleLMeX uses. keyvals may
-   be NULL. The current HIP device is the one the reactor runs on. */
+   "drm19", "syn53"; reactor_type: 2 (constant pressure, rho h) is the one PeleLMeX uses.
+   keyvals may be NULL. The current HIP device is the one the reactor runs on.
+   Threads: calls on one handle may come from several host threads (PeleLMeX's OpenMP loop over
+   boxes, Source/PeleLMeX_Reactions.cpp:48-55); they are serialized per handle, since its
+   buffers and statistics are shared. Separate handles run independently. pr_last_error is
+   per thread. */
 int pr_create(const char* integrator, const char* mechanism, int reactor_type, int ncells,
               const char* keyvals, pr_handle** out);
 int pr_set_typ_vals(pr_handle* h, const double* typ, int n);

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -386,6 +387,9 @@ struct Surrogate {
 };
 
 struct pr_handle {
+  // Calls on one handle are serialized (its buffers, counters and statistics are shared state);
+  // recursive because entry points call each other (pr_advance_chemistry_box -> pr_react_box).
+  mutable std::recursive_mutex mu;
   int batch_surr = 0;            // the last pr_react_batch_enqueue ran the learned map (pr_collect reads its counters)
   unsigned int batch_acap = 0;   // and audited up to this many cells
   const pr_mech_t* mech = nullptr;
@@ -464,6 +468,7 @@ int pr_create(const char* integrator, const char* mechanism, int reactor_type, i
 
 int pr_set_typ_vals(pr_handle* h, const double* typ, int n) {
   if (!h || !typ) return fail(PR_ERR_ARG, "NULL argument");
+  std::lock_guard<std::recursive_mutex> lock(h->mu);
   if (n != h->mech->nsp + 1) return fail(PR_ERR_ARG, "typical values need NUM_SPECIES + 1 entries");
   std::vector<double> at(n);
   for (int i = 0; i < n; ++i) {
@@ -1020,6 +1025,7 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
                  const pr_array4* rHsrc, const pr_array4* FC, const pr_iarray4* mask,
                  double* dt_react, double* time, void* stream, int* status_out) {
   if (!h || !lo || !hi || !dt_react) return fail(PR_ERR_ARG, "NULL argument");
+  std::lock_guard<std::recursive_mutex> lock(h->mu);
   const int N = h->mech->nsp;
   if (!arr_ok(rY, N) || !arr_ok(rYsrc, N) || !arr_ok(T, 1) || !arr_ok(rH, 1) || !arr_ok(rHsrc, 1) || !arr_ok(FC, 1))
     return fail(PR_ERR_ARG, "missing array");
@@ -1163,6 +1169,7 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
 
 int pr_react_batch_enqueue(pr_handle* h, int nbox, const pr_box* boxes, void* stream) {
   if (!h || nbox < 1 || !boxes) return fail(PR_ERR_ARG, "bad argument");
+  std::lock_guard<std::recursive_mutex> lock(h->mu);
   const int N = h->mech->nsp;
   std::vector<prk::BoxDesc> descs((size_t)nbox);
   std::vector<long> starts((size_t)nbox + 1, 0L);
@@ -1287,6 +1294,7 @@ int pr_react_batch_enqueue(pr_handle* h, int nbox, const pr_box* boxes, void* st
 
 int pr_collect(pr_handle* h, void* stream) {
   if (!h) return fail(PR_ERR_ARG, "NULL argument");
+  std::lock_guard<std::recursive_mutex> lock(h->mu);
   HIPCHK(hipSetDevice(h->device));
   hipStream_t st = (hipStream_t)stream;
   auto* base = static_cast<unsigned char*>(h->d_counters);
@@ -1319,8 +1327,10 @@ int pr_collect(pr_handle* h, void* stream) {
 }
 
 int pr_react_batch(pr_handle* h, int nbox, const pr_box* boxes, void* stream) {
+  if (!h) return fail(PR_ERR_ARG, "NULL argument");
+  std::lock_guard<std::recursive_mutex> lock(h->mu);
   hipStream_t st = (hipStream_t)stream;
-  if (h) HIPCHK(hipEventRecord(h->ev0, st));
+  HIPCHK(hipEventRecord(h->ev0, st));
   int rc = pr_react_batch_enqueue(h, nbox, boxes, stream);
   if (rc) return rc;
   HIPCHK(hipEventRecord(h->ev1, st));
@@ -1337,6 +1347,7 @@ static int eval_batch(pr_handle* h, int mode, int64_t ncells, const double* y, c
                       const double* rhoh0, const double* rhohsrc, double t, double* out, void* stream) {
   if (!h || !y || !rYsrc || !rhohsrc || !out || (mode == 0 && !rhoh0) || ncells <= 0)
     return fail(PR_ERR_ARG, "bad argument");
+  std::lock_guard<std::recursive_mutex> lock(h->mu);
   HIPCHK(hipSetDevice(h->device));
   const int G = h->var->G;
   const int block = pick_block(h);
@@ -1373,6 +1384,7 @@ int pr_reaction_rate_box(pr_handle* h, const int lo[3], const int hi[3], const p
                          const pr_array4* rH, const pr_array4* T, const pr_array4* rYdot,
                          const pr_iarray4* mask, void* stream) {
   if (!h || !lo || !hi) return fail(PR_ERR_ARG, "NULL argument");
+  std::lock_guard<std::recursive_mutex> lock(h->mu);
   const int N = h->mech->nsp;
   if (!arr_ok(rY, N) || !arr_ok(rH, 1) || !arr_ok(T, 1) || !arr_ok(rYdot, N)) return fail(PR_ERR_ARG, "missing array");
   if ((rY->ncomp && rY->ncomp < N) || (rYdot->ncomp && rYdot->ncomp < N))
@@ -1422,6 +1434,7 @@ int pr_reaction_rate_box(pr_handle* h, const int lo[3], const int hi[3], const p
 
 // Launches one of the EOS box kernels (one group per cell, grid-stride) and synchronizes.
 static int eos_launch(pr_handle* h, int mode, const int lo[3], const int hi[3], prk::EosParams& P, void* stream) {
+  std::lock_guard<std::recursive_mutex> lock(h->mu);
   const long nx = (long)hi[0] - lo[0] + 1, ny = (long)hi[1] - lo[1] + 1, nz = (long)hi[2] - lo[2] + 1;
   if (nx <= 0 || ny <= 0 || nz <= 0) return fail(PR_ERR_ARG, "empty box");
   const long ncells = nx * ny * nz;
@@ -1454,6 +1467,7 @@ static int eos_launch(pr_handle* h, int mode, const int lo[3], const int hi[3], 
 int pr_box_costs(pr_handle* h, int nbox, const int* lo, const int* hi, const pr_array4* FC, int method,
                  double* costs, void* stream) {
   if (!h || nbox < 0 || (nbox > 0 && (!lo || !hi || !FC || !costs))) return fail(PR_ERR_ARG, "NULL argument");
+  std::lock_guard<std::recursive_mutex> lock(h->mu);
   if (method < prk::COST_NCELL || method > prk::COST_FC_SUM) return fail(PR_ERR_ARG, "unknown cost method");
   if (nbox == 0) return 0;
   std::vector<prk::CostBox> bx((size_t)nbox);
@@ -1539,6 +1553,7 @@ int pr_reaction_source_box(pr_handle* h, const int lo[3], const int hi[3], const
                            const pr_array4* rY_new, const pr_array4* extF_rY, const pr_array4* I_R, double dt,
                            void* stream) {
   if (!h || !lo || !hi) return fail(PR_ERR_ARG, "NULL argument");
+  std::lock_guard<std::recursive_mutex> lock(h->mu);
   const int N = h->mech->nsp;
   if (!arr_ok(rY_old, N) || !arr_ok(rY_new, N) || !arr_ok(extF_rY, N) || !arr_ok(I_R, N))
     return fail(PR_ERR_ARG, "missing array");
@@ -1579,6 +1594,7 @@ int pr_advance_chemistry_box(pr_handle* h, const int lo[3], const int hi[3], con
                              const pr_array4* extF_rH, const pr_array4* FC, const pr_iarray4* mask,
                              const pr_array4* I_R, double dt, void* stream, int* status_out) {
   if (!h || !lo || !hi) return fail(PR_ERR_ARG, "NULL argument");
+  std::lock_guard<std::recursive_mutex> lock(h->mu);
   const int N = h->mech->nsp;
   if (!arr_ok(rY_old, N) || !arr_ok(rH_old, 1) || !arr_ok(T_old, 1) || !arr_ok(rY_new, N) ||
       !arr_ok(rH_new, 1) || !arr_ok(T_new, 1) || !arr_ok(extF_rY, N) || !arr_ok(extF_rH, 1) || !arr_ok(FC, 1))
@@ -1630,6 +1646,7 @@ int pr_advance_chemistry_box(pr_handle* h, const int lo[3], const int hi[3], con
 int pr_surrogate_load(pr_handle* h, int nlayers, const int* dims, const float* params, const float* xnorm,
                       const float* ynorm) {
   if (!h || !dims || !params || !xnorm || !ynorm || nlayers < 1 || nlayers > 64) return fail(PR_ERR_ARG, "bad argument");
+  std::lock_guard<std::recursive_mutex> lock(h->mu);
   const int N = h->mech->nsp;
   if (dims[0] != N + 2 || dims[nlayers] != N + 1)
     return fail(PR_ERR_ARG, "surrogate needs NUM_SPECIES + 2 inputs and NUM_SPECIES + 1 outputs");
@@ -1745,6 +1762,7 @@ int pr_surrogate_load(pr_handle* h, int nlayers, const int* dims, const float* p
 
 int pr_surrogate_forward(pr_handle* h, int64_t ncells, const void* X, float* Y, void* stream) {
   if (!h || !X || !Y || ncells < 0) return fail(PR_ERR_ARG, "bad argument");
+  std::lock_guard<std::recursive_mutex> lock(h->mu);
   if (!h->surr) return fail(PR_ERR_ARG, "no network loaded");
   if (ncells == 0) return 0;
   if (ncells > 0x7fffffffL) return fail(PR_ERR_ARG, "too many rows");
@@ -1764,6 +1782,7 @@ int pr_surrogate_forward(pr_handle* h, int64_t ncells, const void* X, float* Y, 
 int pr_surrogate_features(pr_handle* h, int64_t ncells, const double* rY, const double* T, const double* dt,
                           float* raw, void* X, void* stream) {
   if (!h || !rY || !T || !dt || ncells <= 0 || (!raw && !X)) return fail(PR_ERR_ARG, "bad argument");
+  std::lock_guard<std::recursive_mutex> lock(h->mu);
   if (X && !h->surr) return fail(PR_ERR_ARG, "normalised features need a loaded surrogate");
   HIPCHK(hipSetDevice(h->device));
   prk::FeatParams F{};
@@ -1861,6 +1880,7 @@ int pr_mlp_to_bf16(int64_t n, const float* a, void* b, void* stream) {
 
 int pr_get_stats(const pr_handle* h, pr_stats* out) {
   if (!h || !out) return fail(PR_ERR_ARG, "NULL argument");
+  std::lock_guard<std::recursive_mutex> lock(h->mu);
   *out = h->last;
   return 0;
 }
