@@ -113,7 +113,7 @@ def main():
     ap.add_argument("--waves", type=int, default=0,
                     help="syn53_128: hipbdf.waves_per_simd (with --block; 0: the autotuner chooses)")
     ap.add_argument("--block", type=int, default=0, help="syn53_128: hipbdf.block (with --waves)")
-    ap.add_argument("--workload", default="drm19_64", choices=["drm19_64", "syn53_128", "amr_mixed"],
+    ap.add_argument("--workload", default="drm19_64", choices=["drm19_64", "syn53_128", "amr_mixed", "h2_32"],
                     help="syn53_128: plain GPU BDF on the synthetic 53-species mechanism, 128^3 cells at 60 atm "
                          "in eight 262144-cell launches (one step = all eight)")
     args = ap.parse_args()
@@ -126,6 +126,9 @@ def main():
         return
     if args.workload == "amr_mixed":
         bench_amr(args)
+        return
+    if args.workload == "h2_32":
+        bench_h2(args)
         return
 
     import torch
@@ -252,6 +255,63 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(box, min(args.cpu_sample, ncells))
+    print(json.dumps(line), flush=True)
+
+
+def bench_h2(args):
+    """Plain GPU BDF on the H2/air LiDryer box (32^3 = 32768 cells, phi ~ U[0.5, 2], burn fraction
+    ~ U[0, 1], radicals 1e-8..1e-3, T ~ U[1000, 1800] K, 1 atm, dt_react 1e-6 s, no sources,
+    rtol = atol = 1e-10). One step = one react over the box restored from a pristine copy (FC
+    carried); the CPU oracle integrates the whole box for the baseline."""
+    import torch
+    from pelelmex_amd import roofline, workloads
+    from pelelmex_amd.reactor import Reactor, box_tensors
+    box = workloads.h2_air_box()
+    n = box["T"].size
+    t = box_tensors(box, torch.device("cuda:0"))
+    pristine = {k: t[k].clone() for k in ("rY", "T", "rH")}
+    r = Reactor.create("ReactorCvode", "LiDryer", {"ode.rtol": 1e-10, "ode.atol": 1e-10,
+                                                   "cvode.max_order": args.max_order, "cvode.max_substeps": 10000})
+    r.init(2, 1)
+
+    def step():
+        for k in pristine:
+            t[k].copy_(pristine[k])
+        nfail, _, _ = r.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], t["mask"], box["dt"], 0.0)
+        if nfail:
+            raise RuntimeError(f"{nfail} cells failed to integrate")
+        return r.stats()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    r.close()
+    st = stats[-1]
+    kernel_ms = float(np.mean([x["kernel_ms"] for x in stats]))
+    F = roofline.flops("LiDryer", st)
+    achieved = F / (kernel_ms / 1e3) / 1e12
+    line = {
+        "metric": "chem cells/s per dt_react: plain GPU BDF, H2/air LiDryer",
+        "value": n / (ms / 1e3), "unit": "cells/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "LiDryer H2/air box 32^3 = 32768 cells, dt_react 1e-6 s, 1 atm, no sources",
+                   "mechanism": "LiDryer", "cells": n, "dt_react": box["dt"], "rtol": 1e-10, "atol": 1e-10,
+                   "max_order": args.max_order, "lanes_per_cell": st["lanes_per_cell"],
+                   "waves_per_simd": st["waves_per_simd"], "seed": 1001},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": roofline.MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / roofline.MI355X_FP64_TFLOPS, "traffic": None,
+                     "traffic_source": "not profiled for this workload",
+                     "kernel": "react_kernel<9,21,%d,%d>" % (st["lanes_per_cell"], st["waves_per_simd"]),
+                     "kernel_ms": kernel_ms, "flops_per_launch": F},
+        "integrator": {"steps_per_cell": st["nst"] / st["cells"], "rhs_per_cell": st["nfe"] / st["cells"]},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(box, n, what="the same 32^3 box (every cell)")
     print(json.dumps(line), flush=True)
 
 
