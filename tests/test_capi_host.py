@@ -130,3 +130,13 @@ template int pele_hip::ReactorHipBDF::advance_chemistry<Box, Array4<double>, Arr
                     "-L", libdir, "-lpelelmex_amd", f"-Wl,-rpath,{libdir}"], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True)
     assert "ode.rtol" in out.stdout
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and says so, before touching a GPU."""
+    import subprocess
+    import sys
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 2
+    assert "one MI355X" in out.stderr
