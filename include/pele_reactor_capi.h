@@ -126,9 +126,10 @@ typedef struct {
 int pr_react_batch(pr_handle* h, int nbox, const pr_box* boxes, void* stream);
 /* pr_react_batch in two halves, for callers that capture the work in a hipGraph: _enqueue only
    enqueues on `stream` (a copy of the box table from the handle's pinned staging memory, the
-   counter reset kernels, the learned map's kernels when one is loaded, the integration and
-   audit kernels: no synchronization, no memset node, and no allocation once a first call of
-   the same or larger size has run); pr_collect waits for the
+   counter reset kernels, the learned map's kernels when one is loaded, the cost-ordering
+   kernels (hipbdf.cost_order), the integration and audit kernels: no synchronization, no
+   memset node, and no allocation once a first call of the same or larger size has run);
+   pr_collect waits for the
    stream and returns the failed-cell count of the last integration, filling pr_get_stats. A
    captured batch replays the box table staged at capture; other react calls on the same handle
    restage it. */
@@ -189,7 +190,8 @@ int pr_reaction_source_box(pr_handle* h, const int lo[3], const int hi[3], const
    state (:101-108; the learned map too when loaded); the new state and the forcing scaled back
    (:110-120); and, when I_R->p is not NULL, I_R = -(rhoY_old - rhoY_new) * (1 / dt) - F_rhoY from
    the restored forcing (:141-160). Every product is the reference's, so the caller's forcing
-   makes the same rounding round trip. Returns the failed-cell count (>= 0) or an error code. */
+   makes the same rounding round trip. Returns the failed-cell count (>= 0) or an error code;
+   after an error inside react the new state and the forcing are still scaled back to MKS. */
 int pr_advance_chemistry_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY_old,
                              const pr_array4* rH_old, const pr_array4* T_old, const pr_array4* rY_new,
                              const pr_array4* rH_new, const pr_array4* T_new, const pr_array4* extF_rY,

@@ -1636,10 +1636,17 @@ int pr_advance_chemistry_box(pr_handle* h, const int lo[3], const int hi[3], con
   double dt_react = dt, time = 0.0;
   const int nfail = pr_react_box(h, lo, hi, rY_new, extF_rY, T_new, rH_new, extF_rH, FC, mask, &dt_react, &time,
                                  stream, status_out);
-  if (nfail < 0) return nfail;
+  // the scaling back runs after a failed react too, so the caller's forcing is restored
+  const std::string react_err = nfail < 0 ? g_err : std::string();
   hipLaunchKernelGGL(prk::chem_units_kernel<true>, dim3(grid), dim3(256), 0, st, P);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(st));
+  const hipError_t e1 = hipGetLastError();
+  const hipError_t e2 = hipStreamSynchronize(st);
+  if (nfail < 0) {
+    g_err = react_err;
+    return nfail;
+  }
+  if (e1 != hipSuccess) return fail(PR_ERR_HIP, std::string("chem_units_kernel: ") + hipGetErrorString(e1));
+  if (e2 != hipSuccess) return fail(PR_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e2));
   return nfail;
 }
 

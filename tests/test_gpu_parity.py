@@ -617,3 +617,31 @@ def test_one_handle_from_two_host_threads(gpu):
     for ts, tp in zip(seq, par):
         for k in ("rY", "T", "rH", "FC"):
             _assert_bits(k, tp[k].cpu().numpy(), ts[k].cpu().numpy())
+
+
+def test_advance_chemistry_error_restores_units(gpu):
+    """An error inside react (here a block size above the variant's limit, found before any
+    integration) still scales the new state and the forcing back to MKS: the forcing ends as
+    the reference's round trip F * 1e-3 * 1e3, the new state as the old one's round trip."""
+    import torch
+    from pelelmex_amd.reactor import Reactor, ReactorError
+    b = W.ch4_air_flame_box(nx=8, ny=4, nz=2, seed=62, dt=3e-6)
+    N, n = b["rY"].shape
+    nx, ny, nz = b["dims"]
+    dev = lambda a, c: torch.as_tensor(np.ascontiguousarray(a)).reshape((c, nz, ny, nx) if c else (nz, ny, nx)).to(gpu)
+    old = {"rY": dev(b["rY"] * 1e3, N), "rH": dev(b["rH"] * 0.1, 0), "T": dev(b["T"], 0)}
+    new = {k: torch.zeros_like(v) for k, v in old.items()}
+    F_rY, F_rH = b["rYsrc"] * 1e3, b["rHsrc"] * 0.1
+    FY, FH = dev(F_rY, N), dev(F_rH, 0)
+    FC = torch.zeros_like(old["T"])
+    r = Reactor.create("ReactorCvode", "drm19", dict(KEYS, **{"hipbdf.lanes_per_cell": 64, "hipbdf.block": 768}))
+    r.init(2, 1)
+    with pytest.raises(ReactorError, match="exceeds"):
+        r.advance_chemistry(old, new, FY, FH, FC, None, b["dt"])
+    torch.cuda.synchronize()
+    r.close()
+    host = lambda t, c: t.cpu().numpy().reshape((c, n) if c else (n,))
+    np.testing.assert_array_equal(host(FY, N), (F_rY * 1.0e-3) * 1.0e3)
+    np.testing.assert_array_equal(host(FH, 0), (F_rH * 10.0) * 0.1)
+    np.testing.assert_array_equal(host(new["rY"], N), ((b["rY"] * 1e3) * 1.0e-3) * 1.0e3)
+    np.testing.assert_array_equal(host(new["T"], 0), b["T"])
