@@ -9,8 +9,8 @@
 //   m_reactor->react(bx, rhoY, frcExt, T, rhoH, frcEExt, fcl, mask, dt, time, stream)
 //                                                          Source/PeleLMeX_Reactions.cpp:101-108
 //   m_reactor->close();                                    Source/PeleLMeX.cpp:26
-// A PelePhysics ReactorBase subclass registered under a factory name forwards these calls to
-// this class.
+// A PelePhysics ReactorBase subclass registered under a factory name (ReactorBase::Register)
+// forwards these calls to this class, which holds the C-ABI handle.
 #ifndef PELE_REACTOR_HIP_H
 #define PELE_REACTOR_HIP_H
 
