@@ -118,7 +118,7 @@ def main():
                          "in eight 262144-cell launches (one step = all eight)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one MI355X; this build has no multi-GPU path (each GPU would run its own "
+        print("bench.py measures one MI355X; the library has no multi-GPU path (each GPU would run its own "
               "boxes independently).", file=sys.stderr)
         sys.exit(2)
     if args.workload == "syn53_128":
