@@ -645,3 +645,25 @@ def test_advance_chemistry_error_restores_units(gpu):
     np.testing.assert_array_equal(host(FH, 0), (F_rH * 10.0) * 0.1)
     np.testing.assert_array_equal(host(new["rY"], N), ((b["rY"] * 1e3) * 1.0e-3) * 1.0e3)
     np.testing.assert_array_equal(host(new["T"], 0), b["T"])
+
+
+def test_h2_air_whole_box_parity(gpu):
+    """The whole LiDryer 32^3 box (every one of its 32768 cells) bit for bit with the oracle."""
+    box = W.h2_air_box()
+    ref = _oracle(box, max_order=4)
+    out = _gpu(box, dict(KEYS, **{"cvode.max_order": 4}), gpu)
+    assert out["nfail"] == 0 and np.all(out["status"] == 0)
+    for k in ("rY", "T", "rH", "FC"):
+        _assert_bits(k, out[k], ref[k])
+
+
+def test_ch4_flame_large_sample_parity(gpu):
+    """4096 cells of a drm19 flame box of another seed (all three slab types, with sources), the
+    default 32-lane build, bit for bit with the oracle."""
+    full = W.ch4_air_flame_box(seed=2077)
+    box = _sample(full, 4096, seed=13)
+    ref = _oracle(box, max_order=4)
+    out = _gpu(box, dict(KEYS, **{"cvode.max_order": 4}), gpu)
+    assert out["nfail"] == 0 and np.all(out["status"] == 0)
+    for k in ("rY", "T", "rH", "FC"):
+        _assert_bits(k, out[k], ref[k])
