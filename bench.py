@@ -78,22 +78,58 @@ def copy_probe(mb, dev):
     del a, b
 
 
-def cpu_baseline(box, sample, seed=7, what="the same 64^3 box"):
-    """CPU oracle on `sample` random cells of the box; returns the cpu_baseline object."""
+def host_cpu():
+    """CPU model, logical CPUs and OMP_NUM_THREADS of the host the baseline runs on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1, os.environ.get("OMP_NUM_THREADS")
+
+
+def cpu_baseline(box, sample, seed=7, what="the same 64^3 box", max_order=4):
+    """CPU oracle on `sample` random cells of the box. Returns (cpu_baseline object, (cell
+    indices, oracle result)) - the result also checks the GPU's on those cells."""
     from oracle import pyoracle as po
     from pelelmex_amd import workloads as W
     rng = np.random.default_rng(seed)
     idx = np.sort(rng.choice(box["T"].size, sample, replace=False))
     sub = W.subsample(box, idx)
     m = po.Mech(box["mech"])
-    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    model, ncpu, omp = host_cpu()
+    nthreads = int(omp or 0) or ncpu
     t0 = time.perf_counter()
     out = m.react(sub["rY"], sub["rYsrc"], sub["T"], sub["rH"], sub["rHsrc"], sub["mask"], sub["dt"],
-                  max_order=4, nthreads=nthreads)
+                  max_order=max_order, nthreads=nthreads)
     dt = time.perf_counter() - t0
-    return {"value": sample / dt, "unit": "cells/s", "cores": nthreads, "kind": "port",
-            "sample": f"{sample} cells (seeded random order) of {what}, CPU oracle BDF (C, OpenMP "
-                      f"dynamic over cells), {dt:.1f} s, failures {int(np.sum(out['status'] != 0))}"}
+    base = {"value": sample / dt, "unit": "cells/s", "cores": nthreads, "kind": "port",
+            "cpu_model": model, "nproc": ncpu, "OMP_NUM_THREADS": omp,
+            "sample": f"{sample} cells (seeded random order) of {what}, CPU oracle BDF (C, -O2, OpenMP "
+                      f"dynamic over cells on {nthreads} threads), {dt:.1f} s, failures "
+                      f"{int(np.sum(out['status'] != 0))}"}
+    return base, (idx, out)
+
+
+def gpu_vs_oracle(gpu_state, ref):
+    """Parity of a GPU react result (device tensors rY [N, ...], T, FC over the whole box) with
+    the CPU oracle's result on the cells it integrated: bit-exactness and the worst errors."""
+    idx, out = ref
+    N = out["rY"].shape[0]
+    gY = gpu_state["rY"].reshape(N, -1).cpu().numpy()[:, idx]
+    gT = gpu_state["T"].reshape(-1).cpu().numpy()[idx]
+    gF = gpu_state["FC"].reshape(-1).cpu().numpy()[idx]
+    act = out["status"] == 0
+    Yg = gY / gY.sum(0)
+    Yo = out["rY"] / out["rY"].sum(0)
+    return {"cells": int(idx.size), "reference": "CPU oracle (same CVODE-semantics BDF, rtol = atol = 1e-10)",
+            "bit_exact": bool(np.array_equal(gY, out["rY"]) and np.array_equal(gT, out["T"]) and
+                              np.array_equal(gF, out["FC"])),
+            "max_dT_rel": float((np.abs(gT - out["T"]) / out["T"])[act].max()),
+            "max_dY": float(np.abs(Yg - Yo)[:, act].max())}
 
 
 def main():
@@ -239,7 +275,7 @@ def main():
         "bdf_value": bdf_value,
         "bdf_ms_per_step": ms_bdf,
         "surrogate": sur,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": roofline.MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": roofline.MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / roofline.MI355X_FP64_TFLOPS, "traffic": traffic,
                      "traffic_source": (f"profiles/{traffic_src} (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a "
                                         "1 GiB device copy)") if traffic else "no profile of these kernel sources",
@@ -254,7 +290,8 @@ def main():
                        "deterministic_repeat": deterministic},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(box, min(args.cpu_sample, ncells))
+        line["cpu_baseline"], ref = cpu_baseline(box, min(args.cpu_sample, ncells), max_order=args.max_order)
+        line["parity"] = gpu_vs_oracle(bdf_state, ref)
     print(json.dumps(line), flush=True)
 
 
@@ -303,7 +340,7 @@ def bench_h2(args):
                    "mechanism": "LiDryer", "cells": n, "dt_react": box["dt"], "rtol": 1e-10, "atol": 1e-10,
                    "max_order": args.max_order, "lanes_per_cell": st["lanes_per_cell"],
                    "waves_per_simd": st["waves_per_simd"], "seed": 1001},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": roofline.MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": roofline.MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / roofline.MI355X_FP64_TFLOPS, "traffic": None,
                      "traffic_source": "not profiled for this workload",
                      "kernel": "react_kernel<9,21,%d,%d>" % (st["lanes_per_cell"], st["waves_per_simd"]),
@@ -311,7 +348,9 @@ def bench_h2(args):
         "integrator": {"steps_per_cell": st["nst"] / st["cells"], "rhs_per_cell": st["nfe"] / st["cells"]},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(box, n, what="the same 32^3 box (every cell)")
+        line["cpu_baseline"], ref = cpu_baseline(box, n, what="the same 32^3 box (every cell)",
+                                                 max_order=args.max_order)
+        line["parity"] = gpu_vs_oracle(t, ref)
     print(json.dumps(line), flush=True)
 
 
@@ -415,7 +454,7 @@ def bench_amr(args):
         "surrogate": {"accepted": st_sur["sur_accepted"] / total, "rejected_to_bdf": st_sur["sur_rejected"] / total,
                       "audited": st_sur["sur_audited"], "audit_max_dT_rel": st_sur["audit_max_dT_rel"],
                       "max_dT_rel_vs_bdf_all_cells": dT, "speedup_vs_bdf": ms_bdf / ms_sur},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": roofline.MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": roofline.MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / roofline.MI355X_FP64_TFLOPS, "traffic": None,
                      "traffic_source": "not profiled for this workload",
                      "kernel": "react_kernel<21,84,32,3> (plain batched BDF, whole graph replay)",
@@ -499,7 +538,7 @@ def bench_syn53(args):
                    "lanes_per_cell": st["lanes_per_cell"], "waves_per_simd": st["waves_per_simd"],
                    "block_threads": st["block_threads"], "seed": "3003 + chunk",
                    "path": "plain GPU BDF (" + ("fixed variant" if args.waves else "autotuned") + ")"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": roofline.MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": roofline.MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / roofline.MI355X_FP64_TFLOPS, "traffic": traffic,
                      "traffic_source": (f"profiles/{traffic_src} (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a "
                                         "1 GiB device copy)") if traffic else "no profile of these kernel sources",
@@ -514,8 +553,8 @@ def bench_syn53(args):
                        "newton_per_cell": tot["nni"] / tot["cells"], "err_fail_per_cell": tot["netf"] / tot["cells"]},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(host0, min(args.cpu_sample, 65536),
-                                            what="the first 64^3 chunk (masked cells count as cells)")
+        line["cpu_baseline"] = cpu_baseline(host0, min(args.cpu_sample, 65536), max_order=args.max_order,
+                                            what="the first 64^3 chunk (masked cells count as cells)")[0]
     print(json.dumps(line), flush=True)
 
 

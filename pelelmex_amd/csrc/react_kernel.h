@@ -1102,6 +1102,17 @@ __global__ void zero_u32(unsigned int* p, int n) {
   for (int t = threadIdx.x; t < n; t += blockDim.x) p[t] = 0u;
 }
 
+// Up to kStoreWords 8-byte words carried in the kernel arguments, stored to dst[0..n): host data
+// staged this way is part of the launch itself, so a captured graph replays the values of its
+// capture whatever the host memory holds later.
+constexpr int kStoreWords = 256;
+struct StoreWords {
+  unsigned long long w[kStoreWords];
+};
+__global__ void store_words(unsigned long long* dst, int n, StoreWords src) {
+  for (int t = threadIdx.x; t < n; t += blockDim.x) dst[t] = src.w[t];
+}
+
 __global__ void zero_counters(unsigned int* queue, unsigned long long* stats, unsigned int* nfail) {
   int t = threadIdx.x;
   if (t < 8) stats[t] = 0ull;

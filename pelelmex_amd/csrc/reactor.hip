@@ -412,9 +412,23 @@ struct pr_handle {
   unsigned int* d_hist = nullptr;  // its bucket counts and cursors (2 x COST_BUCKETS)
   long order_cap = 0;
   void* d_boxtab = nullptr;  // device box table (descriptors, then cell prefix sums)
-  void* h_boxtab = nullptr;  // pinned staging copy of it
   size_t boxtab_bytes = 0;
+  // Set once pr_react_batch_enqueue has been captured into a graph: the graph's nodes hold this
+  // handle's device buffers, so buffers a later call outgrows are retired (freed at pr_destroy)
+  // instead of freed, and the graph keeps replaying on the ones it captured.
+  bool captured = false;
+  std::vector<void*> retired;
 };
+
+// Frees a device buffer of the handle, or keeps it until pr_destroy once a captured graph may
+// reference it.
+static void release(pr_handle* h, void* p) {
+  if (!p) return;
+  if (h->captured)
+    h->retired.push_back(p);
+  else
+    (void)hipFree(p);
+}
 
 extern "C" {
 
@@ -454,8 +468,12 @@ int pr_create(const char* integrator, const char* mechanism, int reactor_type, i
   }
   std::vector<unsigned char> blob = pack_blob(m);
   h->blob_bytes = (int)blob.size();
+  // per-component absolute tolerances: ode.atol everywhere until set_typ_vals_ode scales them
+  // (the kernels always read this array, so a captured graph sees later typical values)
+  const std::vector<double> atol0((size_t)m->nsp + 1, K.atol);
   if (hipMalloc(&h->d_blob, blob.size()) != hipSuccess || hipMalloc(&h->d_counters, 256) != hipSuccess ||
       hipMalloc(&h->d_atolv, sizeof(double) * (m->nsp + 1)) != hipSuccess ||
+      hipMemcpy(h->d_atolv, atol0.data(), sizeof(double) * atol0.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->d_blob, blob.data(), blob.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
       hipEventCreate(&h->ev2) != hipSuccess || hipEventCreate(&h->ev3) != hipSuccess) {
@@ -475,6 +493,9 @@ int pr_set_typ_vals(pr_handle* h, const double* typ, int n) {
     if (!(typ[i] > 0)) return fail(PR_ERR_ARG, "typical values must be positive");
     at[i] = h->keys.atol * typ[i];
   }
+  // no launch of this handle may still be reading the old values (graph replays included)
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipMemcpy(h->d_atolv, at.data(), sizeof(double) * n, hipMemcpyHostToDevice));
   h->use_typ = true;
   return 0;
@@ -513,7 +534,7 @@ static int bdf_block(const pr_handle* h) {
 static int ensure_gscr(pr_handle* h, long slots) {
   const size_t need = (size_t)slots * (size_t)h->var->slot;
   if (need <= h->gscr_bytes) return 0;
-  if (h->d_gscr) (void)hipFree(h->d_gscr);
+  release(h, h->d_gscr);
   h->d_gscr = nullptr;
   h->gscr_bytes = 0;
   HIPCHK(hipMalloc(&h->d_gscr, need));
@@ -567,25 +588,31 @@ static int bdf_launch(pr_handle* h, prk::KParams& P, long work_max, hipStream_t 
   return 0;
 }
 
-// Copies box descriptors and their cell prefix sums to the device table (through pinned
-// staging memory, reused after the call's final stream synchronization).
+// Copies box descriptors and their cell prefix sums to the device table. The bytes travel in
+// the arguments of store_words launches (no host staging buffer), so a captured launch sequence
+// replays the table of its capture even after other calls restage the handle's table.
 static int stage_boxes(pr_handle* h, const std::vector<prk::BoxDesc>& d, const std::vector<long>& start,
                        hipStream_t st, const prk::BoxDesc** dd, const long** ds) {
   const size_t nd = d.size() * sizeof(prk::BoxDesc), ns = start.size() * sizeof(long);
   const size_t off = (nd + 255) / 256 * 256, bytes = off + ns;
   if (bytes > h->boxtab_bytes) {
-    if (h->d_boxtab) (void)hipFree(h->d_boxtab);
-    if (h->h_boxtab) (void)hipHostFree(h->h_boxtab);
-    h->d_boxtab = h->h_boxtab = nullptr;
+    release(h, h->d_boxtab);
+    h->d_boxtab = nullptr;
     h->boxtab_bytes = 0;
     HIPCHK(hipMalloc(&h->d_boxtab, bytes));
-    HIPCHK(hipHostMalloc(&h->h_boxtab, bytes, hipHostMallocDefault));
     h->boxtab_bytes = bytes;
   }
-  auto* hb = static_cast<unsigned char*>(h->h_boxtab);
-  std::memcpy(hb, d.data(), nd);
-  std::memcpy(hb + off, start.data(), ns);
-  HIPCHK(hipMemcpyAsync(h->d_boxtab, h->h_boxtab, bytes, hipMemcpyHostToDevice, st));
+  std::vector<unsigned long long> words((bytes + 7) / 8, 0ull);
+  std::memcpy(words.data(), d.data(), nd);
+  std::memcpy(reinterpret_cast<unsigned char*>(words.data()) + off, start.data(), ns);
+  auto* dst = static_cast<unsigned long long*>(h->d_boxtab);
+  for (size_t w0 = 0; w0 < words.size(); w0 += prk::kStoreWords) {
+    const int n = (int)std::min<size_t>(prk::kStoreWords, words.size() - w0);
+    prk::StoreWords chunk;
+    std::memcpy(chunk.w, words.data() + w0, sizeof(unsigned long long) * (size_t)n);
+    hipLaunchKernelGGL(prk::store_words, dim3(1), dim3(256), 0, st, dst + w0, n, chunk);
+  }
+  HIPCHK(hipGetLastError());
   *dd = static_cast<const prk::BoxDesc*>(h->d_boxtab);
   *ds = reinterpret_cast<const long*>(static_cast<unsigned char*>(h->d_boxtab) + off);
   return 0;
@@ -602,16 +629,20 @@ static int ensure_surr_buffers(pr_handle* h, long ncells) {
   Surrogate* S = h->surr;
   if (ncells <= S->cap) return 0;
   const int N = h->mech->nsp;
-  for (void* q : {(void*)S->X, (void*)S->H0, (void*)S->H1, (void*)S->Yout, (void*)S->Tstar, (void*)S->scale, (void*)S->rej, (void*)S->aud_in,
-                  (void*)S->aud_out, (void*)S->aud_box, (void*)S->aud_err, (void*)S->aud_cell, (void*)S->aud_dt})
-    if (q) {
-      (void)hipFree(q);
+  void** bufs[] = {(void**)&S->X, (void**)&S->H0, (void**)&S->H1, (void**)&S->Yout, (void**)&S->Tstar,
+                   (void**)&S->scale, (void**)&S->rej, (void**)&S->aud_in, (void**)&S->aud_out, (void**)&S->aud_box,
+                   (void**)&S->aud_err, (void**)&S->aud_cell, (void**)&S->aud_dt};
+  for (void** q : bufs) {
+    if (*q) {
+      release(h, *q);
       for (auto it = S->owned.begin(); it != S->owned.end(); ++it)
-        if (*it == q) {
+        if (*it == *q) {
           S->owned.erase(it);
           break;
         }
     }
+    *q = nullptr;  // no freed address survives a failed reallocation below
+  }
   S->cap = 0;
   int hid = 1;
   for (size_t l = 1; l + 1 < S->dims.size(); ++l) hid = S->dims[l] > hid ? S->dims[l] : hid;
@@ -798,7 +829,7 @@ static int autotune(pr_handle* h, const prk::KParams& Pbase, const prk::BoxDesc&
 // listed count P.ncells_dev is unchanged). No memset and no allocation once sized for ncells.
 static int cost_order_launch(pr_handle* h, prk::KParams& P, long ncells, hipStream_t st) {
   if (h->order_cap < ncells) {
-    if (h->d_order) (void)hipFree(h->d_order);
+    release(h, h->d_order);
     h->d_order = nullptr;
     h->order_cap = 0;
     HIPCHK(hipMalloc(&h->d_order, sizeof(int) * (size_t)ncells));
@@ -1065,7 +1096,7 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
   P.one.dt = dt;
   P.rtol = h->keys.rtol;
   P.atol = h->keys.atol;
-  P.atolv = h->use_typ ? h->d_atolv : nullptr;
+  P.atolv = h->d_atolv;
   P.max_order = h->keys.max_order;
   P.max_steps = h->keys.max_substeps;
   P.clean = h->keys.clean_init_massfrac;
@@ -1214,12 +1245,17 @@ int pr_react_batch_enqueue(pr_handle* h, int nbox, const pr_box* boxes, void* st
   if (total > 0x7fffffffL) return fail(PR_ERR_ARG, "too many cells");
   HIPCHK(hipSetDevice(h->device));
   hipStream_t st = (hipStream_t)stream;
+  {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) h->captured = true;
+  }
   prk::KParams P{};
   P.one = descs[0];
   P.ncells = total;
   P.rtol = h->keys.rtol;
   P.atol = h->keys.atol;
-  P.atolv = h->use_typ ? h->d_atolv : nullptr;
+  P.atolv = h->d_atolv;
   P.max_order = h->keys.max_order;
   P.max_steps = h->keys.max_substeps;
   P.clean = h->keys.clean_init_massfrac;
@@ -1760,7 +1796,7 @@ int pr_surrogate_load(pr_handle* h, int nlayers, const int* dims, const float* p
       hipMemcpy(S->Minv, Minv.data(), Minv.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(PR_ERR_HIP, "upload failed");
   if (h->surr) {
-    for (void* q : h->surr->owned) (void)hipFree(q);
+    for (void* q : h->surr->owned) release(h, q);
     delete h->surr;
   }
   h->surr = S;
@@ -1903,8 +1939,8 @@ void pr_destroy(pr_handle* h) {
   if (h->ev2) (void)hipEventDestroy(h->ev2);
   if (h->ev3) (void)hipEventDestroy(h->ev3);
   if (h->d_boxtab) (void)hipFree(h->d_boxtab);
-  if (h->h_boxtab) (void)hipHostFree(h->h_boxtab);
   if (h->d_order) (void)hipFree(h->d_order);
+  for (void* q : h->retired) (void)hipFree(q);
   if (h->d_hist) (void)hipFree(h->d_hist);
   if (h->surr) {
     for (void* q : h->surr->owned) (void)hipFree(q);

@@ -413,6 +413,29 @@ class Reactor:
             pass
 
 
+def typical_values_chem(rY_mks, T, Ymin=1.0e-10):
+    """The typical values PeleLMeX hands the reactor (set_typ_vals_ode), from a state in MKS as
+    the level MultiFabs hold it: rY_mks [N, ncells] (kg m^-3), T [ncells] (K).
+
+    PeleLM::setTypicalValues (Source/PeleLMeX_Utils.cpp:1363-1392): rho_typ = (max rho + min rho)/2,
+    Y_typ,k = (max rhoY_k + min rhoY_k) / 2 / rho_typ, T_typ = (max T + min T) / 2 (rho = sum_k rhoY_k
+    here; PeleLMeX keeps it as its own component). PeleLM::updateTypicalValuesChem (:1444-1475):
+    typ_k = max(Ymin rho_typ 1e-3, Y_typ,k rho_typ 1e-3) (g cm^-3), typ_N = T_typ, with Ymin =
+    peleLM.typical_values_Ymin (default 1e-10, Source/PeleLMeX.H:2003). The reactor then integrates
+    with atol_i = ode.atol * typ_i (ode.atol is the pre-factor of the typical values,
+    Docs/sphinx/manual/LMeXControls.rst:304)."""
+    import numpy as np
+    rY_mks = np.asarray(rY_mks, dtype=np.float64)
+    T = np.asarray(T, dtype=np.float64)
+    rho = rY_mks.sum(0)
+    rho_typ = 0.5 * (rho.max() + rho.min())
+    Y_typ = 0.5 * (rY_mks.max(1) + rY_mks.min(1)) / rho_typ
+    typ = np.empty(rY_mks.shape[0] + 1)
+    typ[:-1] = np.maximum(Ymin * rho_typ * 1.0e-3, Y_typ * rho_typ * 1.0e-3)
+    typ[-1] = 0.5 * (T.max() + T.min())
+    return typ
+
+
 def box_tensors(box, device="cuda"):
     """Device tensors in Array4 order for a workload dict from pelelmex_amd.workloads."""
     import torch
