@@ -32,8 +32,9 @@
 #include "../pelelmex_amd/csrc/mech/lidryer.h"
 #include "../pelelmex_amd/csrc/mech/drm19.h"
 #include "../pelelmex_amd/csrc/mech/syn53.h"
+#include "../pelelmex_amd/csrc/mech/dodecane_lu.h"
 
-static const pr_mech_t* const MECHS[] = {&pr_lidryer, &pr_drm19, &pr_syn53};
+static const pr_mech_t* const MECHS[] = {&pr_lidryer, &pr_drm19, &pr_syn53, &pr_dodecane_lu};
 #define NMECH ((int)(sizeof(MECHS) / sizeof(MECHS[0])))
 
 int orc_mech_count(void) { return NMECH; }

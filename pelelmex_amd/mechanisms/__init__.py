@@ -13,7 +13,7 @@ RU = 8.31446261815324e7   # erg / (mol K)
 PATM = 1013250.0          # dyn / cm^2
 ATOMIC_WEIGHTS = {"H": 1.008, "C": 12.011, "O": 15.999, "N": 14.007, "Ar": 39.95}
 
-_MODULES = {"lidryer": "lidryer", "drm19": "drm19", "syn53": "syn53"}
+_MODULES = {"lidryer": "lidryer", "drm19": "drm19", "syn53": "syn53", "dodecane_lu": "dodecane_lu"}
 _CACHE = {}
 
 

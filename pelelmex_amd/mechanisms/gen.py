@@ -29,7 +29,7 @@ import sys
 
 ATOMIC_WEIGHTS = {"H": 1.008, "C": 12.011, "O": 15.999, "N": 14.007, "Ar": 39.95}
 R_CAL = 8.314462618 / 4.184  # cal / (mol K)
-MECHANISMS = ["lidryer", "drm19", "syn53"]
+MECHANISMS = ["lidryer", "drm19", "syn53", "dodecane_lu"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 OUT_DIR = os.path.join(os.path.dirname(_HERE), "csrc", "mech")
@@ -188,12 +188,16 @@ def emit(name):
             t = 3 if r.troe is not None else 2
         rtype.append(t)
         rev.append(1 if r.rev else 0)
-        if len(r.reac) > 3 or len(r.prod) > 3:
-            raise ValueError(f"more than 3 molecules per side in {r.eq}")
+        # up to 3 molecules per side are stored (the slots the reverse rate reads); an
+        # irreversible reaction may list up to 6 products, which enter the production rates
+        # through the incidence lists only
+        if len(r.reac) > 3 or len(r.prod) > (3 if r.rev else 6):
+            raise ValueError(f"too many molecules on a side of {r.eq}")
         nreac.append(len(r.reac))
         reac += [idx[s] for s in r.reac] + [0] * (3 - len(r.reac))
-        nprod.append(len(r.prod))
-        prod += [idx[s] for s in r.prod] + [0] * (3 - len(r.prod))
+        ps = r.prod[:3]
+        nprod.append(len(ps))
+        prod += [idx[s] for s in ps] + [0] * (3 - len(ps))
         arr += [math.log(r.A), r.b, r.E / R_CAL]
         if r.low is not None:
             low += [math.log(r.low[0]), r.low[1], r.low[2] / R_CAL]
