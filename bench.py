@@ -147,21 +147,24 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device copy of this size (calibrates byte counters in a profile)")
     ap.add_argument("--waves", type=int, default=0,
-                    help="syn53_128: hipbdf.waves_per_simd (with --block; 0: the autotuner chooses)")
-    ap.add_argument("--block", type=int, default=0, help="syn53_128: hipbdf.block (with --waves)")
-    ap.add_argument("--workload", default="drm19_64", choices=["drm19_64", "syn53_128", "amr_mixed", "h2_32"],
-                    help="syn53_128: plain GPU BDF on the synthetic 53-species mechanism, 128^3 cells at 60 atm "
-                         "in eight 262144-cell launches (one step = all eight)")
+                    help="dodecane_128 / syn53_128: hipbdf.waves_per_simd (with --block; 0: the autotuner chooses)")
+    ap.add_argument("--block", type=int, default=0, help="dodecane_128 / syn53_128: hipbdf.block (with --waves)")
+    ap.add_argument("--workload", default="drm19_64",
+                    choices=["drm19_64", "dodecane_128", "syn53_128", "amr_mixed", "amr_dodecane", "h2_32"],
+                    help="dodecane_128: plain GPU BDF on n-dodecane (config 3), 128^3 cells at 60 atm in eight "
+                         "262144-cell launches (one step = all eight); syn53_128: the same on the synthetic "
+                         "53-species mechanism; amr_mixed / amr_dodecane: learned map + batched fallback over "
+                         "mixed AMR boxes in one hipGraph (drm19 / n-dodecane, config 5)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X; the library has no multi-GPU path (each GPU would run its own "
               "boxes independently).", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "syn53_128":
-        bench_syn53(args)
+    if args.workload in ("dodecane_128", "syn53_128"):
+        bench_chunks(args, "dodecane_lu" if args.workload == "dodecane_128" else "syn53")
         return
-    if args.workload == "amr_mixed":
-        bench_amr(args)
+    if args.workload in ("amr_mixed", "amr_dodecane"):
+        bench_amr(args, "dodecane_lu" if args.workload == "amr_dodecane" else "drm19")
         return
     if args.workload == "h2_32":
         bench_h2(args)
@@ -354,15 +357,16 @@ def bench_h2(args):
     print(json.dumps(line), flush=True)
 
 
-def bench_amr(args):
-    """Learned map + batched BDF fallback over mixed AMR boxes in one hipGraph (drm19): boxes of
+def bench_amr(args, mech="drm19"):
+    """Learned map + batched BDF fallback over mixed AMR boxes in one hipGraph (config 5): boxes of
     three levels with edge 16, 32 or 64 drawn uniformly until about 1.5 M cells, per-box dt_react
-    from {1e-6, 2e-6, 4e-6} s, flame-like states with 5 % Gaussian sources (seed 5005 + box). One
-    step = restoring every box's state from its pristine copy (FC carries over, as PeleLMeX's
-    functC does, so from the second step the cells are handed out by decreasing cost) and one
-    replay of the captured pr_react_batch_enqueue over all boxes. The plain batched BDF, captured
-    the same way, is timed on the same boxes, and the learned result is compared with it on every
-    cell."""
+    from {1e-6, 2e-6, 4e-6} s (seed 5005 + box). mech "dodecane_lu": config-3 n-dodecane states
+    (60 atm, 10 % masked) with the 6 x 512 network; mech "drm19": flame-like drm19 states with 5 %
+    Gaussian sources and the 4 x 256 network. One step = restoring every box's state from its
+    pristine copy (FC carries over, as PeleLMeX's functC does, so from the second step the cells are
+    handed out by decreasing cost) and one replay of the captured pr_react_batch_enqueue over all
+    boxes. The plain batched BDF, captured the same way, is timed on the same boxes, and the learned
+    result is compared with it on every cell."""
     import torch
     from pelelmex_amd import roofline, surrogate, workloads
     from pelelmex_amd.reactor import Reactor, box_tensors
@@ -373,23 +377,28 @@ def bench_amr(args):
         e = int(rng.choice([16, 32, 64]))
         lev = int(rng.integers(0, 3))
         dt = float(rng.choice([1e-6, 2e-6, 4e-6]))
-        b = workloads.ch4_air_flame_box(nx=e, ny=e, nz=e, seed=5005 + len(boxes), dt=dt)
+        if mech == "dodecane_lu":
+            b = workloads.dodecane_box(nx=e, ny=e, nz=e, seed=5005 + len(boxes), dt=dt)
+        else:
+            b = workloads.ch4_air_flame_box(nx=e, ny=e, nz=e, seed=5005 + len(boxes), dt=dt)
         b["level"] = lev
         boxes.append(b)
         total += e ** 3
     base = [box_tensors(b, dev) for b in boxes]
-    print(f"amr_mixed: {len(boxes)} boxes, {total} cells", file=sys.stderr, flush=True)
-    wpath = os.path.join(ROOT, "pelelmex_amd", "surrogate_drm19_amr.npz")
-    model = surrogate.load(wpath)
+    print(f"amr ({mech}): {len(boxes)} boxes, {total} cells", file=sys.stderr, flush=True)
+    wname = "surrogate_dodecane_amr.npz" if mech == "dodecane_lu" else "surrogate_drm19_amr.npz"
+    wpath = args.surrogate or os.path.join(ROOT, "pelelmex_amd", wname)
+    model = surrogate.load(wpath) if os.path.exists(wpath) else None
+    lanes = 64 if mech == "dodecane_lu" else 32
     keys = {"ode.rtol": 1e-10, "ode.atol": 1e-10, "cvode.max_order": args.max_order, "cvode.max_substeps": 10000,
-            "cvode.solve_type": "denseAJ_direct", "hipbdf.lanes_per_cell": 32}
+            "cvode.solve_type": "denseAJ_direct", "hipbdf.lanes_per_cell": lanes}
 
     def timed(with_model):
         skeys = dict(keys)
         if with_model:
             skeys["surrogate.trust_rtol"] = float(model["trust_rtol"])
             skeys["surrogate.trust_atol"] = float(model["trust_atol"])
-        r = Reactor.create("ReactorCvode", "drm19", skeys)
+        r = Reactor.create("ReactorCvode", mech, skeys)
         r.init(2, 1)
         if with_model:
             surrogate.attach(r, model)
@@ -433,50 +442,62 @@ def bench_amr(args):
         return ms, st, out
 
     ms_bdf, st_bdf, o_bdf = timed(False)
-    ms_sur, st_sur, o_sur = timed(True)
-    dT = max(((a - b).abs() / b).max().item() for a, b in zip(o_sur["T"], o_bdf["T"]))
-    F = roofline.flops("drm19", st_bdf)
+    F = roofline.flops(mech, st_bdf)
     achieved = F / (ms_bdf / 1e3) / 1e12
+    counts = roofline.static_counts(mech)
     sizes = {e: sum(1 for b in boxes if b["dims"][0] == e) for e in (16, 32, 64)}
+    sur = None
+    if model is not None:
+        ms_sur, st_sur, o_sur = timed(True)
+        act = [torch.as_tensor(b["mask"] != -1, device=dev).reshape(t.shape) for b, t in zip(boxes, o_bdf["T"])]
+        dT = max(((a - b).abs() / b)[m].max().item() for a, b, m in zip(o_sur["T"], o_bdf["T"], act))
+        sur = {"value": total / (ms_sur / 1e3), "ms_per_step": ms_sur,
+               "accepted": st_sur["sur_accepted"] / total, "rejected_to_bdf": st_sur["sur_rejected"] / total,
+               "audited": st_sur["sur_audited"], "audit_max_dT_rel": st_sur["audit_max_dT_rel"],
+               "max_dT_rel_vs_bdf_all_cells": dT, "speedup_vs_bdf": ms_bdf / ms_sur,
+               "network": os.path.relpath(wpath, ROOT) + " (%s)" % " x ".join(str(x) for x in model["dims"][1:-1])}
+    value = sur["value"] if sur else total / (ms_bdf / 1e3)
+    what = "n-dodecane (dodecane_lu), 60 atm, 10% masked" if mech == "dodecane_lu" else "drm19, 5% sources"
     line = {
-        "metric": "chem cells/s per dt_react: surrogate+fallback over mixed AMR boxes in one hipGraph (drm19)",
-        "value": total / (ms_sur / 1e3), "unit": "cells/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_sur, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"drm19, {len(boxes)} boxes on 3 levels (edges 16/32/64: {sizes[16]}/{sizes[32]}/"
-                               f"{sizes[64]}), {total} cells, per-box dt_react in {{1e-6, 2e-6, 4e-6}} s, 5% sources",
-                   "mechanism": "drm19", "cells": total, "boxes": len(boxes), "max_order": args.max_order,
-                   "lanes_per_cell": 32, "seed": "5005 + box",
-                   "network": "pelelmex_amd/surrogate_drm19_amr.npz (4 x 256, trained over dt 1e-6 / 2e-6 / 4e-6)",
-                   "path": "one hipGraph: features per box, one network pass, trust check per box, one persistent "
-                           "BDF launch over every box's rejected cells, 1% audit"},
+        "metric": f"chem cells/s per dt_react: surrogate+fallback over mixed AMR boxes in one hipGraph ({mech})",
+        "value": value, "unit": "cells/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sur["ms_per_step"] if sur else ms_bdf, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{what}; {len(boxes)} boxes on 3 levels (edges 16/32/64: {sizes[16]}/{sizes[32]}/"
+                               f"{sizes[64]}), {total} cells, per-box dt_react in {{1e-6, 2e-6, 4e-6}} s",
+                   "mechanism": mech, "cells": total, "boxes": len(boxes), "max_order": args.max_order,
+                   "lanes_per_cell": lanes, "seed": "5005 + box",
+                   "path": ("one hipGraph: features per box, one network pass, trust check per box, one persistent "
+                            "BDF launch over every box's rejected cells, 1% audit") if sur else
+                           "one hipGraph: one persistent BDF launch over every box (no network shipped)"},
         "bdf_value": total / (ms_bdf / 1e3), "bdf_ms_per_step": ms_bdf,
-        "surrogate": {"accepted": st_sur["sur_accepted"] / total, "rejected_to_bdf": st_sur["sur_rejected"] / total,
-                      "audited": st_sur["sur_audited"], "audit_max_dT_rel": st_sur["audit_max_dT_rel"],
-                      "max_dT_rel_vs_bdf_all_cells": dT, "speedup_vs_bdf": ms_bdf / ms_sur},
+        "surrogate": sur,
         "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": roofline.MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / roofline.MI355X_FP64_TFLOPS, "traffic": None,
                      "traffic_source": "not profiled for this workload",
-                     "kernel": "react_kernel<21,84,32,3> (plain batched BDF, whole graph replay)",
+                     "kernel": "react_kernel<%d,%d,%d,*> (plain batched BDF, whole graph replay)" % (
+                         counts["N"], counts["R"], lanes),
                      "flops_per_step": F},
     }
     print(json.dumps(line), flush=True)
 
 
-def bench_syn53(args):
-    """Plain GPU BDF at the n-dodecane size: syn53 (synthetic 53 species, 268 reactions), a 128^3 box
-    at 60 atm with 10 % of cells masked, dt_react 2e-6 s, as eight 64^3 chunks of 262144 cells (one
-    react launch each). One step = the eight launches over the whole box, each chunk restored from
-    a pristine device copy first. Lanes per cell, waves per SIMD and block size are chosen by the
-    reactor's autotuner on the first (warm-up) call, or fixed by --waves / --block."""
+def bench_chunks(args, mech="dodecane_lu"):
+    """Plain GPU BDF on config 3: n-dodecane (dodecane_lu, 53 species, 253 reactions; or the
+    synthetic syn53 of the same size with mech="syn53"), a 128^3 box at 60 atm with 10 % of cells
+    masked, dt_react 2e-6 s, as eight 64^3 chunks of 262144 cells (one react launch each). One step
+    = the eight launches over the whole box, each chunk restored from a pristine device copy first.
+    Lanes per cell, waves per SIMD and block size are chosen by the reactor's autotuner on the first
+    (warm-up) call, or fixed by --waves / --block."""
     import torch
     from pelelmex_amd import roofline, workloads
     from pelelmex_amd.reactor import Reactor, box_tensors
     dev = torch.device("cuda:0")
+    gen_chunk = workloads.dodecane_chunk if mech == "dodecane_lu" else workloads.syn53_chunk
     nchunk = 8
     chunks, pristine, host0 = [], [], None
     for c in range(nchunk):
-        b = workloads.syn53_chunk(c)
+        b = gen_chunk(c)
         if c == 0:
             host0 = b
         t = box_tensors(b, dev)
@@ -493,7 +514,7 @@ def bench_syn53(args):
             keys["hipbdf.block"] = args.block
     else:
         keys["hipbdf.autotune"] = 1
-    r = Reactor.create("ReactorCvode", "syn53", keys)
+    r = Reactor.create("ReactorCvode", mech, keys)
     r.init(2, 1)
 
     def launch(c):
@@ -505,7 +526,7 @@ def bench_syn53(args):
             raise RuntimeError(f"{nfail} cells failed to integrate")
         return r.stats()
 
-    print(f"syn53: {ncells} cells generated and resident; warm-up (autotune) on chunk 0", file=sys.stderr, flush=True)
+    print(f"{mech}: {ncells} cells generated and resident; warm-up (autotune) on chunk 0", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         launch(0)
     torch.cuda.synchronize()
@@ -518,22 +539,25 @@ def bench_syn53(args):
     r.close()
     tot = {k: sum(s[k] for s in stats) for k in ("nfe", "nje", "nsetups", "nni", "nst", "netf", "cells")}
     kernel_ms = sum(s["kernel_ms"] for s in stats)
-    F = roofline.flops("syn53", tot)
+    F = roofline.flops(mech, tot)
     achieved = F / (kernel_ms / 1e3) / 1e12
-    counts = roofline.static_counts("syn53")
+    counts = roofline.static_counts(mech)
     st = stats[-1]
-    workload_key = f"syn53 128^3 lanes {st['lanes_per_cell']} waves {st['waves_per_simd']} max_order {args.max_order}"
+    workload_key = f"{mech} 128^3 lanes {st['lanes_per_cell']} waves {st['waves_per_simd']} max_order {args.max_order}"
     traffic, traffic_src = profiled_traffic(workload_key)
+    what = ("n-dodecane dodecane_lu (53 species, 253 reactions)" if mech == "dodecane_lu"
+            else "syn53 (synthetic: 53 species, 268 reactions)")
     line = {
-        "metric": "chem cells/s per dt_react: plain GPU BDF, synthetic 53-species mechanism (n-dodecane size)",
+        "metric": "chem cells/s per dt_react: plain GPU BDF, " + ("n-dodecane" if mech == "dodecane_lu"
+                                                                  else "synthetic 53-species mechanism"),
         "value": ncells / (ms / 1e3),
         "unit": "cells/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "syn53 (synthetic: 53 species, 268 reactions) 128^3 = 2097152 cells in 8 launches of "
+        "config": {"workload": what + " 128^3 = 2097152 cells in 8 launches of "
                                "262144, 60 atm, dt_react 2e-6 s, 10% masked, mixture fraction ~ Beta(0.5, 2), "
                                "T ~ U[700, 2400] K",
-                   "mechanism": "syn53", "cells": ncells, "active_cells": active, "dt_react": host0["dt"],
+                   "mechanism": mech, "cells": ncells, "active_cells": active, "dt_react": host0["dt"],
                    "rtol": 1e-10, "atol": 1e-10, "max_order": args.max_order,
                    "lanes_per_cell": st["lanes_per_cell"], "waves_per_simd": st["waves_per_simd"],
                    "block_threads": st["block_threads"], "seed": "3003 + chunk",
@@ -542,8 +566,8 @@ def bench_syn53(args):
                      "frac": achieved / roofline.MI355X_FP64_TFLOPS, "traffic": traffic,
                      "traffic_source": (f"profiles/{traffic_src} (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a "
                                         "1 GiB device copy)") if traffic else "no profile of these kernel sources",
-                     "kernel": "react_kernel<53,268,%d,%d> (%d-thread blocks)" % (
-                         st["lanes_per_cell"], st["waves_per_simd"], st["block_threads"]),
+                     "kernel": "react_kernel<53,%d,%d,%d> (%d-thread blocks)" % (
+                         counts["R"], st["lanes_per_cell"], st["waves_per_simd"], st["block_threads"]),
                      "launches_per_step": nchunk, "kernel_ms_per_step": kernel_ms,
                      "kernel_ms_avg_launch": kernel_ms / nchunk, "flops_per_step": F,
                      "traffic_per": "launch", "hbm_algorithmic_bytes_per_launch": counts["bytes_cell"] * ncells // nchunk,

@@ -11,6 +11,7 @@
  *                    parses from ParmParse (Docs/sphinx/manual/LMeXControls.rst:300-308) are
  *                    passed as "key = value" text.
  *   pr_set_typ_vals  ReactorBase::set_typ_vals_ode (Source/PeleLMeX_Utils.cpp:1472)
+ *   pr_set_eos_parm  ReactorBase::set_eos_parm (Source/PeleLMeX_Setup.cpp:155)
  *   pr_react_box     ReactorBase::react(box, rY, rYsrc, T, rE, rEsrc, FC, mask, dt, time, stream)
  *                    (Source/PeleLMeX_Reactions.cpp:101-108 and :267-274)
  *   pr_react_batch   the per-box loop around react in advanceChemistry
@@ -34,6 +35,8 @@
  *   pr_floor_species_box    PeleLM::floorSpecies per box (Source/PeleLMeX_Utils.cpp:649-690)
  *   pr_advance_chemistry_box  the per-box body of PeleLM::advanceChemistry(lev, dt, Forcing)
  *                    (Source/PeleLMeX_Reactions.cpp:54-160: conversions, react, I_R)
+ *   pr_advance_chemistry_chem_box  the per-box body of PeleLM::advanceChemistryBAChem on the
+ *                    chemistry BoxArray (Source/PeleLMeX_Reactions.cpp:220-305)
  *   pr_reaction_source_box  I_R formation in advanceChemistry (Source/PeleLMeX_Reactions.cpp:141-160,
  *                    342-347)
  *   pr_heat_release_box     PeleLM::getHeatRelease per box (Source/PeleLMeX_Reactions.cpp:477-506,
@@ -89,7 +92,7 @@ typedef struct {
 #define PR_ERR_KEY (-4)
 
 /* integrator: "ReactorCvode" (drop-in name) or "ReactorHipBDF"; mechanism: "LiDryer",
-   "drm19", "syn53"; reactor_type: 2 (constant pressure, rho h) is the one PeleLMeX uses.
+   "drm19", "dodecane_lu", "syn53"; reactor_type: 2 (constant pressure, rho h) is the one PeleLMeX uses.
    keyvals may be NULL. The current HIP device is the one the reactor runs on.
    Threads: calls on one handle may come from several host threads (PeleLMeX's OpenMP loop over
    boxes, Source/PeleLMeX_Reactions.cpp:48-55); they are serialized per handle, since its
@@ -197,6 +200,27 @@ int pr_advance_chemistry_box(pr_handle* h, const int lo[3], const int hi[3], con
                              const pr_array4* rH_new, const pr_array4* T_new, const pr_array4* extF_rY,
                              const pr_array4* extF_rH, const pr_array4* FC, const pr_iarray4* mask,
                              const pr_array4* I_R, double dt, void* stream, int* status_out);
+/* One box of PeleLM::advanceChemistryBAChem(lev, dt, Forcing) on the chemistry BoxArray's copy
+   of the state (Source/PeleLMeX_Reactions.cpp:220-305), in place on MKS views: rY (NUM_SPECIES
+   components), rH and T of the chem copy and its forcing copy extF_rY, extF_rH. rhoY and its
+   forcing times 1e-3, rho h and its forcing times 10 (:228-238); when do_reaction != 0
+   (m_baChemFlag: the box is not entirely EB-covered) pr_react_box over dt (:260-274), otherwise
+   FC = 0 on every cell and no integration (:276-280); then the state only back to MKS (rhoY times
+   1e3, rho h times 0.1, :284-291; the forcing copy stays in CGS, as the reference leaves its
+   scratch copy). The ParallelCopy into and out of the chem copy and the I_R of :342-347
+   (pr_reaction_source_box) stay with the caller. Returns the failed-cell count (>= 0) or an
+   error code; after an error inside react the state is still scaled back. */
+int pr_advance_chemistry_chem_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY,
+                                  const pr_array4* rH, const pr_array4* T, const pr_array4* extF_rY,
+                                  const pr_array4* extF_rH, const pr_array4* FC, const pr_iarray4* mask,
+                                  int do_reaction, double dt, void* stream, int* status_out);
+/* ReactorBase::set_eos_parm (Source/PeleLMeX_Setup.cpp:155, called right after init with
+   eos_parms.device_parm()): eos_parm is the caller's EOS parameter block (device pointer, may be
+   NULL), eos_model the build's Eos_Model ("Fuego", the ideal-gas mixture EOS of the reference
+   cases, or "GammaLaw"; NULL means "Fuego"). The reactor's thermodynamics is the ideal-gas mixture
+   of the mechanism's NASA-7 fits, for which that block carries no parameter it reads: the pointer
+   is recorded and not dereferenced. A real-gas model (e.g. "SRK") returns PR_ERR_UNSUPPORTED. */
+int pr_set_eos_parm(pr_handle* h, const void* eos_parm, const char* eos_model);
 /* Learned dt_react map. Network: nlayers dense layers, dims[0] = NUM_SPECIES + 2 inputs
    (log10(Y_k + 1e-20), T / 2000, log10(dt)), dims[nlayers] = NUM_SPECIES + 1 outputs (dY_k, dT),
    GELU between layers. params (host, fp32): for each layer W [dims[l] x dims[l+1]] row-major then

@@ -5,6 +5,7 @@
 // Reference call sites this adapter serves (PeleLMeX):
 //   m_reactor = ReactorBase::create(m_chem_integrator);   Source/PeleLMeX_Setup.cpp:152-153
 //   m_reactor->init(reactor_type = 2, ncells_chem = 1);    Source/PeleLMeX_Setup.cpp:154
+//   m_reactor->set_eos_parm(eos_parms.device_parm());      Source/PeleLMeX_Setup.cpp:155
 //   m_reactor->set_typ_vals_ode(typical_values);           Source/PeleLMeX_Utils.cpp:1472
 //   m_reactor->react(bx, rhoY, frcExt, T, rhoH, frcEExt, fcl, mask, dt, time, stream)
 //                                                          Source/PeleLMeX_Reactions.cpp:101-108
@@ -73,6 +74,14 @@ class ReactorHipBDF {
     return 0;
   }
 
+  // ReactorBase::set_eos_parm(eos_parms.device_parm()) (Source/PeleLMeX_Setup.cpp:155). The
+  // reactor's EOS is the ideal-gas mixture (Fuego) of the mechanism: the block is recorded, not read.
+  template <class EosParm>
+  void set_eos_parm(EosParm const* parm, const char* eos_model = "Fuego") {
+    int rc = pr_set_eos_parm(h_, static_cast<const void*>(parm), eos_model);
+    if (rc != 0) throw ReactorError(std::string("pr_set_eos_parm: ") + pr_last_error());
+  }
+
   void set_typ_vals_ode(const std::vector<double>& typ) {
     int rc = pr_set_typ_vals(h_, typ.data(), static_cast<int>(typ.size()));
     if (rc != 0) throw ReactorError(std::string("pr_set_typ_vals: ") + pr_last_error());
@@ -108,6 +117,22 @@ class ReactorHipBDF {
     int rc = pr_advance_chemistry_box(h_, lo, hi, &a[0], &a[1], &a[2], &a[3], &a[4], &a[5], &a[6], &a[7], &a[8], &m,
                                       &a[9], dt, stream, nullptr);
     if (rc < 0) throw ReactorError(std::string("pr_advance_chemistry_box: ") + pr_last_error());
+    return rc;
+  }
+
+  // The per-box body of PeleLM::advanceChemistryBAChem (Source/PeleLMeX_Reactions.cpp:220-305) on
+  // the chemistry BoxArray's copy (MKS, in place): do_reaction is m_baChemFlag[lev][mfi.index()].
+  template <class Box, class A4, class I4>
+  int advance_chemistry_chem(const Box& box, A4 const& rhoY, A4 const& rhoH, A4 const& temp, A4 const& extF_rhoY,
+                             A4 const& extF_rhoH, A4 const& fcl, I4 const& mask, int do_reaction, double dt,
+                             void* stream = nullptr) {
+    const int lo[3] = {box.smallEnd(0), box.smallEnd(1), box.smallEnd(2)};
+    const int hi[3] = {box.bigEnd(0), box.bigEnd(1), box.bigEnd(2)};
+    pr_array4 a[6] = {to_pr(rhoY), to_pr(rhoH), to_pr(temp), to_pr(extF_rhoY), to_pr(extF_rhoH), to_pr(fcl)};
+    pr_iarray4 m = to_pr_mask(mask);
+    int rc = pr_advance_chemistry_chem_box(h_, lo, hi, &a[0], &a[1], &a[2], &a[3], &a[4], &a[5], &m, do_reaction,
+                                           dt, stream, nullptr);
+    if (rc < 0) throw ReactorError(std::string("pr_advance_chemistry_chem_box: ") + pr_last_error());
     return rc;
   }
 

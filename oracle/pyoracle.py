@@ -224,6 +224,25 @@ def advance_chemistry(mech, rY_o, rH_o, T_o, F_rY, F_rH, mask, dt, **react_kw):
                 IR=reaction_source(rY_o, rYn, FY, dt))
 
 
+def advance_chemistry_chem(mech, rY, rH, T, F_rY, F_rH, mask, dt, do_reaction=True, **react_kw):
+    """One box of PeleLM::advanceChemistryBAChem on the chemistry BoxArray's copy (MKS,
+    component-major [N, ncells] / [ncells]; Source/PeleLMeX_Reactions.cpp:220-305): rhoY and its
+    forcing times 1e-3, rho h and its forcing times 10 (:228-238); the oracle's react when
+    do_reaction, else FC = 0 and the state untouched (:260-280); the state (only) back to MKS
+    (:284-291). Returns the new state, the forcing copy (in CGS), FC and status."""
+    rYc = np.asarray(rY, dtype=np.float64) * 1.0e-3
+    FY = np.asarray(F_rY, dtype=np.float64) * 1.0e-3
+    rHc = np.asarray(rH, dtype=np.float64) * 10.0
+    FH = np.asarray(F_rH, dtype=np.float64) * 10.0
+    T = np.array(T, dtype=np.float64)
+    if do_reaction:
+        out = mech.react(rYc, FY, T, rHc, FH, mask, dt, **react_kw)
+        rYc, rHc, T, FC, status = out["rY"], out["rH"], out["T"], out["FC"], out["status"]
+    else:
+        FC, status = np.zeros(T.size), np.zeros(T.size, dtype=np.int32)
+    return dict(rY=rYc * 1.0e3, rH=rHc * 0.1, T=T, FY=FY, FH=FH, FC=FC, status=status)
+
+
 def exp(x):
     return lib().orc_exp(float(x))
 

@@ -208,4 +208,34 @@ __global__ void __launch_bounds__(256) chem_units_kernel(ChemBoxParams P) {
   }
 }
 
+// advanceChemistryBAChem's unit conversions on the chemistry BoxArray's copy, in place
+// (Source/PeleLMeX_Reactions.cpp:228-238 and :284-291): before the react, rhoY and its forcing
+// times 1e-3, rho h and its forcing times 10 (P.rYn, P.rHn, P.FY, P.FH); after it, the state only
+// (rhoY times 1e3, rho h times 0.1: the forcing copy stays in CGS). ZERO_FC additionally writes
+// FC = 0 (a covered box, which skips the react).
+template <int MODE>   // 0 before, 1 after, 2 after with FC = 0
+__global__ void __launch_bounds__(256) chem_units_inplace_kernel(ChemBoxParams P, Arr FC) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < P.ncells; c += stride) {
+    const unsigned u = (unsigned)c;
+    const int i = P.lo0 + (int)(u % (unsigned)P.nx);
+    const int j = P.lo1 + (int)((u / (unsigned)P.nx) % (unsigned)P.ny);
+    const int k = P.lo2 + (int)(u / ((unsigned)P.nx * (unsigned)P.ny));
+    const long oY = aoff(P.rYn, i, j, k);
+    if (MODE == 0) {
+      const long oF = aoff(P.FY, i, j, k);
+      for (int n = 0; n < P.N; ++n) {
+        P.rYn.p[oY + n * P.rYn.ns] *= 1.0e-3;
+        P.FY.p[oF + n * P.FY.ns] *= 1.0e-3;
+      }
+      P.rHn.p[aoff(P.rHn, i, j, k)] *= 10.0;
+      P.FH.p[aoff(P.FH, i, j, k)] *= 10.0;
+    } else {
+      for (int n = 0; n < P.N; ++n) P.rYn.p[oY + n * P.rYn.ns] *= 1.0e3;
+      P.rHn.p[aoff(P.rHn, i, j, k)] *= 0.1;
+      if (MODE == 2) FC.p[aoff(FC, i, j, k)] = 0.0;
+    }
+  }
+}
+
 }  // namespace prk

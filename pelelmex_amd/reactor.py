@@ -24,7 +24,8 @@ _LIB = None
 
 EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_react_batch", "pr_react_batch_enqueue", "pr_collect", "pr_rhs_batch", "pr_jac_batch",
            "pr_reaction_rate_box", "pr_box_costs", "pr_set_temperature_box", "pr_floor_species_box",
-           "pr_heat_release_box", "pr_reaction_source_box", "pr_advance_chemistry_box", "pr_surrogate_load", "pr_surrogate_features", "pr_surrogate_forward", "pr_mlp_gemm",
+           "pr_heat_release_box", "pr_reaction_source_box", "pr_advance_chemistry_box",
+           "pr_advance_chemistry_chem_box", "pr_set_eos_parm", "pr_surrogate_load", "pr_surrogate_features", "pr_surrogate_forward", "pr_mlp_gemm",
            "pr_mlp_gelu_backward", "pr_mlp_column_sum", "pr_mlp_mse", "pr_mlp_adam", "pr_mlp_to_bf16",
            "pr_get_stats", "pr_num_species", "pr_destroy", "pr_last_error"]
 
@@ -98,6 +99,9 @@ def lib():
                                              C.c_double, C.c_void_p]
         L.pr_advance_chemistry_box.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)] + [a4] * 9 + \
             [C.POINTER(IArray4), a4, C.c_double, C.c_void_p, C.c_void_p]
+        L.pr_advance_chemistry_chem_box.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)] + \
+            [a4] * 6 + [C.POINTER(IArray4), C.c_int, C.c_double, C.c_void_p, C.c_void_p]
+        L.pr_set_eos_parm.argtypes = [C.c_void_p, C.c_void_p, C.c_char_p]
         vp, i64, fp = C.c_void_p, C.c_int64, C.POINTER(C.c_float)
         L.pr_surrogate_load.argtypes = [vp, C.c_int, C.POINTER(C.c_int), fp, fp, fp]
         L.pr_surrogate_features.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
@@ -350,6 +354,34 @@ class Reactor:
         if rc < 0:
             raise ReactorError(f"pr_advance_chemistry_box failed ({rc}): {lib().pr_last_error().decode()}")
         return rc
+
+    def advance_chemistry_chem(self, rY, rH, T, F_rY, F_rH, FC, mask, dt, do_reaction=True, stream=None,
+                               status=None):
+        """One box of advanceChemistryBAChem on the chemistry BoxArray's copy (device tensors, MKS,
+        in place): rY [N, nz, ny, nx], rH, T, FC [nz, ny, nx], the forcing copy F_rY / F_rH (left in
+        CGS, as the reference leaves it); do_reaction False = a covered box (FC = 0, no
+        integration). Returns the failed-cell count."""
+        import torch
+        nz, ny, nx = T.shape
+        lo = (C.c_int * 3)(0, 0, 0)
+        hi = (C.c_int * 3)(nx - 1, ny - 1, nz - 1)
+        a = [_array4(rY, self.nsp), _array4(rH, 1), _array4(T, 1), _array4(F_rY, self.nsp), _array4(F_rH, 1),
+             _array4(FC, 1)]
+        m = IArray4(mask.data_ptr(), nx, nx * ny, (C.c_int * 3)(0, 0, 0)) if mask is not None else None
+        st = stream if stream is not None else torch.cuda.current_stream()
+        rc = lib().pr_advance_chemistry_chem_box(self._h, lo, hi, *[C.byref(x) for x in a],
+                                                 C.byref(m) if m is not None else None, int(bool(do_reaction)),
+                                                 C.c_double(float(dt)), C.c_void_p(st.cuda_stream),
+                                                 C.c_void_p(status.data_ptr()) if status is not None else None)
+        if rc < 0:
+            raise ReactorError(f"pr_advance_chemistry_chem_box failed ({rc}): {lib().pr_last_error().decode()}")
+        return rc
+
+    def set_eos_parm(self, eos_parm=None, eos_model="Fuego"):
+        """ReactorBase::set_eos_parm: records the EOS parameter block (device pointer or None)."""
+        rc = lib().pr_set_eos_parm(self._h, eos_parm, eos_model.encode() if eos_model else None)
+        if rc != 0:
+            raise ReactorError(f"pr_set_eos_parm failed ({rc}): {lib().pr_last_error().decode()}")
 
     def load_surrogate(self, dims, params, xnorm, ynorm):
         """pr_surrogate_load: dims (layer widths), params (fp32 W0, b0, W1, b1, ... flattened),

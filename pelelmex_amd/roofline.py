@@ -24,7 +24,7 @@ HY2T_ITERS = 2
 
 
 def _reaction_flops(r, jac):
-    nre, npr = len(r.reac), len(r.prod)
+    nre, npr = len(r.reac), min(len(r.prod), 3)   # the kernel reads 3 product slots at most
     f = 5                                   # k = exp(fma(fma)))
     if r.third is not None:
         f += 1 + 2 * len([e for e in r.eff.values() if e != 1.0])  # [M]

@@ -163,7 +163,9 @@ class MLP:
 
 def workload_box(mech, seed, dt, nx=64, src_frac=0.0):
     """A training or calibration box of the mechanism's workload: the drm19 flame-like box
-    (optionally with sources), or a syn53 60 atm box (no sources, no mask)."""
+    (optionally with sources), or an n-dodecane / syn53 60 atm box (no sources, no mask)."""
+    if mech == "dodecane_lu":
+        return workloads.dodecane_box(nx=nx, ny=nx, nz=nx, seed=seed, dt=dt, masked=0.0)
     if mech == "drm19":
         return workloads.ch4_air_flame_box(nx=nx, ny=nx, nz=nx, seed=seed, dt=dt, src_frac=src_frac)
     if mech == "syn53":
@@ -395,7 +397,7 @@ def main():
     ap.add_argument("--seed", type=int, default=40040)
     ap.add_argument("--target", type=float, default=1e-6)
     ap.add_argument("--model", default=None, help="calibrate this trained network instead of training one")
-    ap.add_argument("--mech", default="drm19", choices=["drm19", "syn53"])
+    ap.add_argument("--mech", default="drm19", choices=["drm19", "dodecane_lu", "syn53"])
     ap.add_argument("--hidden", default="256,256,256,256", help="hidden layer widths")
     ap.add_argument("--dts", default="5e-6", help="dt_react values, cycled over the training and calibration boxes")
     args = ap.parse_args()

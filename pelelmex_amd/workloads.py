@@ -20,6 +20,7 @@ Recipes (documented choices where the configuration text leaves them open):
   dissociation radicals H, O, OH each Y ~ 10^U[-6, -3], T ~ U[1800, 2300] K; renormalised.
   External sources rhoY_src_k = 0.05 |rhoY_k| / dt * xi_k and rhoH_src = 0.05 |rhoH| / dt * xi_h
   with xi ~ N(0, 1); P = 1 atm; dt = 5e-6 s.
+* n-dodecane (dodecane_lu, 128^3 in eight 64^3 chunks, seed 3003 + chunk): see dodecane_box.
 * syn53 at the n-dodecane size (synthetic 53-species mechanism, 128^3 in eight 64^3 chunks,
   seed 3003 + chunk): mixture fraction Z ~ Beta(0.5, 2) between a CH4 fuel stream and air
   (O2:N2 = 1 : 3.76), phi = (Z / (1 - Z)) / (Z_st / (1 - Z_st)) with Z_st = 0.0552; burn
@@ -157,10 +158,67 @@ def syn53_chunk(chunk=0, nx=64, ny=64, nz=64, seed=3003, dt=2e-6, masked=0.10):
     return _finish(m, Y, T, 60.0 * PATM, dt, 0.0, rng, (nx, ny, nz), mask=mask)
 
 
+def _fuel_air_products(m, X_u, fuel, nC, nH):
+    """Complete-combustion products of unburnt mole fractions X_u (fuel CnCHnH + O2 + N2): lean or
+    stoichiometric -> CO2, H2O and leftover O2; rich -> carbon to CO, hydrogen to H2O, the remaining
+    oxygen turning CO into CO2; too rich to oxidize every carbon to CO -> the fuel burns to CO + H2O
+    as far as the oxygen goes and the rest stays fuel. Element-conserving, per unburnt mole."""
+    n = X_u.shape[0]
+    F = X_u[:, m.index[fuel]]
+    O = 2.0 * X_u[:, m.index["O2"]]
+    C, H = nC * F, nH * F
+    X = np.zeros((n, m.nsp))
+    X[:, m.index["N2"]] = X_u[:, m.index["N2"]]
+    lean = O >= 2 * C + H / 2
+    mid = (~lean) & (O >= C + H / 2)
+    rich = ~(lean | mid)
+    X[:, m.index["CO2"]] = np.where(lean, C, np.where(mid, O - C - H / 2, 0.0))
+    X[:, m.index["H2O"]] = np.where(rich, nH / 2 * O / (nC + nH / 2), H / 2)
+    X[:, m.index["O2"]] = np.where(lean, (O - 2 * C - H / 2) / 2, 0.0)
+    X[:, m.index["CO"]] = np.where(mid, C - (O - C - H / 2), np.where(rich, nC * O / (nC + nH / 2), 0.0))
+    X[:, m.index[fuel]] = np.where(rich, F - O / (nC + nH / 2), 0.0)
+    return X
+
+
+def dodecane_box(nx=64, ny=64, nz=64, seed=3003, dt=2e-6, masked=0.10, P_atm=60.0):
+    """n-dodecane box (dodecane_lu; one 64^3 chunk = 262144 cells by default) at 60 atm: mixture
+    fraction Z ~ Beta(0.5, 2) between pure NC12H26 and air (O2 : N2 = 1 : 3.76), burn fraction
+    c ~ U[0, 1] toward the element-conserving complete-combustion products of the same Z
+    (_fuel_air_products), radicals H, O, OH, HO2, H2O2 each Y ~ 10^U[-8, -3], renormalised,
+    T ~ U[700, 2400] K, no sources, dt 2e-6 s, a fraction `masked` of cells covered (mask -1)."""
+    m = load("dodecane_lu")
+    rng = _rng(seed)
+    n = nx * ny * nz
+    Z = rng.beta(0.5, 2.0, n)
+    c = rng.uniform(0.0, 1.0, n)
+    T = rng.uniform(700.0, 2400.0, n)
+    W_F = m.W[m.index["NC12H26"]]
+    W_air = (m.W[m.index["O2"]] + 3.76 * m.W[m.index["N2"]]) / 4.76
+    nF = Z / W_F
+    nA = (1.0 - Z) / W_air
+    X_u = np.zeros((n, m.nsp))
+    X_u[:, m.index["NC12H26"]] = nF / (nF + nA)
+    X_u[:, m.index["O2"]] = nA / (nF + nA) / 4.76
+    X_u[:, m.index["N2"]] = nA / (nF + nA) * 3.76 / 4.76
+    Y_u = m.X_to_Y(X_u)
+    Y_b = m.X_to_Y(_fuel_air_products(m, X_u, "NC12H26", 12, 26))
+    Y = (1 - c)[:, None] * Y_u + c[:, None] * Y_b
+    for s in ("H", "O", "OH", "HO2", "H2O2"):
+        Y[:, m.index[s]] += 10.0 ** rng.uniform(-8.0, -3.0, n)
+    mask = np.where(rng.uniform(0.0, 1.0, n) < masked, -1, 1).astype(np.int32)
+    return _finish(m, Y, T, P_atm * PATM, dt, 0.0, rng, (nx, ny, nz), mask=mask)
+
+
+def dodecane_chunk(chunk=0, nx=64, ny=64, nz=64, seed=3003, dt=2e-6, masked=0.10):
+    """One 64^3 chunk (262144 cells) of the 128^3 n-dodecane box at 60 atm (seed 3003 + chunk)."""
+    return dodecane_box(nx, ny, nz, seed=seed + int(chunk), dt=dt, masked=masked)
+
+
 CONFIGS = {
     "h2_air_32": h2_air_box,
     "ch4_air_flame_64": ch4_air_flame_box,
     "syn53_60atm_64": syn53_chunk,
+    "dodecane_60atm_64": dodecane_chunk,
 }
 
 

@@ -570,6 +570,44 @@ def test_advance_chemistry_box_bits(gpu):
     assert np.all(ref["FC"][cov] == 0) and np.all(ref["FC"][~cov] > 0)
 
 
+@pytest.mark.parametrize("do_reaction", [1, 0])
+def test_advance_chemistry_chem_box_bits(gpu, do_reaction):
+    """pr_advance_chemistry_chem_box (one box of advanceChemistryBAChem on the chemistry BoxArray's
+    copy, in place: conversions, react - or FC = 0 for a box flagged covered - and the state's
+    conversion back, the forcing copy left in CGS) against the same sequence around the oracle's
+    react, bit for bit; set_eos_parm accepts the ideal-gas EOS and refuses a real-gas one."""
+    import torch
+    from oracle import pyoracle as po
+    from pelelmex_amd.reactor import Reactor, ReactorError
+    b = W.ch4_air_flame_box(nx=16, ny=4, nz=2, seed=64, dt=3e-6)
+    b["mask"][::5] = -1
+    N, n = b["rY"].shape
+    rY, rH, T = b["rY"] * 1.0e3, b["rH"] * 0.1, b["T"].copy()
+    F_rY, F_rH = b["rYsrc"] * 1.0e3, b["rHsrc"] * 0.1
+    ref = po.advance_chemistry_chem(po.Mech("drm19"), rY, rH, T, F_rY, F_rH, b["mask"], b["dt"],
+                                    do_reaction=bool(do_reaction), max_order=4)
+    nx, ny, nz = b["dims"]
+    dev = lambda a, c: torch.as_tensor(np.ascontiguousarray(a)).reshape((c, nz, ny, nx) if c else (nz, ny, nx)).to(gpu)
+    tY, tH, tT, FY, FH = dev(rY, N), dev(rH, 0), dev(T, 0), dev(F_rY, N), dev(F_rH, 0)
+    FC = torch.full_like(tT, 7.0)
+    mask = torch.as_tensor(b["mask"]).reshape(nz, ny, nx).to(gpu)
+    r = Reactor.create("ReactorCvode", "drm19", dict(KEYS, **{"cvode.max_order": 4}))
+    r.init(2, 1)
+    r.set_eos_parm(None, "Fuego")
+    with pytest.raises(ReactorError):
+        r.set_eos_parm(None, "SRK")
+    assert r.advance_chemistry_chem(tY, tH, tT, FY, FH, FC, mask, b["dt"], do_reaction=bool(do_reaction)) == 0
+    torch.cuda.synchronize()
+    r.close()
+    host = lambda t, c: t.cpu().numpy().reshape((c, n) if c else (n,))
+    for name, got, want, c in (("rY", tY, ref["rY"], N), ("rH", tH, ref["rH"], 0), ("T", tT, ref["T"], 0),
+                               ("F_rY", FY, ref["FY"], N), ("F_rH", FH, ref["FH"], 0), ("FC", FC, ref["FC"], 0)):
+        _assert_bits(name, host(got, c), want)
+    if not do_reaction:
+        assert np.all(ref["FC"] == 0)
+        np.testing.assert_array_equal(host(tY, N), (rY * 1.0e-3) * 1.0e3)
+
+
 def test_one_handle_from_two_host_threads(gpu):
     """Calls on one handle from two host threads at once (as PeleLMeX's OpenMP box loop would
     make them) are serialized by the library: each thread's boxes get the bits a sequential
