@@ -86,6 +86,8 @@ def test_dodecane_too_much_work_and_clean_init(gpu):
     assert np.all(np.isin(ref["status"][failed], (-1, -4)))      # too much work, and convergence failures
     np.testing.assert_array_equal(out["status"], ref["status"])
     assert out["nfail"] == int(failed.sum())
+    # only a failed cell may end in a non-finite state (here one convergence failure does)
+    assert np.all(np.isfinite(out["rY"][:, ~failed])) and np.all(np.isfinite(out["T"][~failed]))
     for k in ("rY", "T", "rH", "FC"):
         _assert_bits(k, out[k], ref[k])
 
