@@ -110,6 +110,23 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
                  const pr_array4* rYsrc, const pr_array4* T, const pr_array4* rH,
                  const pr_array4* rHsrc, const pr_array4* FC, const pr_iarray4* mask,
                  double* dt_react, double* time, void* stream, int* status_out);
+/* No counterpart in the reference. pr_react_box for callers that cannot allocate device memory
+   (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): same
+   arguments, kernels and results (bit for bit), but it calls no hipMalloc / hipFree. Its
+   counters, box table, cost order and BDF group slots live in one static device array of
+   PR_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends. Limits: a box of at most
+   4194304 cells when hipbdf.cost_order = 1 (larger boxes return PR_ERR_ARG); the persistent
+   grid is capped to the group slots the array holds (on a 256-CU MI355X that is every resident
+   group of every shipped kernel variant); no learned map
+   (surrogate.enable = 1 returns PR_ERR_UNSUPPORTED) and no hipbdf.autotune before a first
+   pr_react_box call has tuned the handle. Every call on one GPU shares that GPU's array, which
+   keeps the last call's contents: calls that overlap on one GPU (two streams, two threads on
+   two handles) race and must be serialized by the caller. */
+#define PR_NOALLOC_SCRATCH_BYTES (192ull << 20)
+int pr_react_box_noalloc(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY,
+                         const pr_array4* rYsrc, const pr_array4* T, const pr_array4* rH,
+                         const pr_array4* rHsrc, const pr_array4* FC, const pr_iarray4* mask,
+                         double* dt_react, double* time, void* stream, int* status_out);
 /* One box of pr_react_batch: the arguments of pr_react_box for that box. */
 typedef struct {
   int lo[3], hi[3];

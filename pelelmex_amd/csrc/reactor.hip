@@ -424,6 +424,8 @@ struct pr_handle {
   const void* eos_parm = nullptr;  // set_eos_parm's block: recorded, not read (ideal-gas EOS)
 };
 
+static int resolve_noalloc_scratch(int device);
+
 // Frees a device buffer of the handle, or keeps it until pr_destroy once a captured graph may
 // reference it.
 static void release(pr_handle* h, void* p) {
@@ -469,6 +471,10 @@ int pr_create(const char* integrator, const char* mechanism, int reactor_type, i
   if (e != hipSuccess) {
     delete h;
     return fail(PR_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  }
+  if (int rc = resolve_noalloc_scratch(h->device)) {
+    delete h;
+    return rc;
   }
   std::vector<unsigned char> blob = pack_blob(m);
   h->blob_bytes = (int)blob.size();
@@ -1089,6 +1095,37 @@ constexpr long kFxMaxOrderCells = 1L << 22;
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
+// The no-allocation react's device scratch (pr_react_box_noalloc): the counters, box table,
+// cost order and BDF group slots of that entry point are carved from this array. Every device
+// the library runs on holds its own copy until the process ends. Host code never passes the
+// array's name to a kernel: its device address is resolved once per device by
+// resolve_noalloc_scratch (from pr_create, with the handle's device current) and kept below.
+constexpr size_t kNoAllocScratchBytes = (size_t)PR_NOALLOC_SCRATCH_BYTES;
+static_assert(kNoAllocScratchBytes > kFxOrder + sizeof(int) * (size_t)kFxMaxOrderCells,
+              "the no-allocation scratch must hold the cost order of its largest box");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::vector<unsigned char*> g_scratch_ptr;  // device address of scratch, per device (null: unresolved)
+
+static int resolve_noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0) return fail(PR_ERR_ARG, "bad device");
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (g_scratch_ptr[(size_t)device]) return 0;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(PR_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[(size_t)device] = static_cast<unsigned char*>(p);
+  return 0;
+}
+
+static unsigned char* noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (device >= 0 && device < (int)g_scratch_ptr.size()) ? g_scratch_ptr[(size_t)device] : nullptr;
+}
+
 // One react call over a box (pr_react_box). fx (nullable) is the no-allocation path's device
 // scratch of fx_bytes bytes: the counters, box table, cost order and group slots are carved from
 // it and nothing is allocated.
@@ -1266,6 +1303,17 @@ int pr_react_box(pr_handle* h, const int lo[3], const int hi[3], const pr_array4
                  double* dt_react, double* time, void* stream, int* status_out) {
   return react_box_core(h, lo, hi, rY, rYsrc, T, rH, rHsrc, FC, mask, dt_react, time, stream, status_out, nullptr,
                         0);
+}
+
+int pr_react_box_noalloc(pr_handle* h, const int lo[3], const int hi[3], const pr_array4* rY,
+                         const pr_array4* rYsrc, const pr_array4* T, const pr_array4* rH,
+                         const pr_array4* rHsrc, const pr_array4* FC, const pr_iarray4* mask,
+                         double* dt_react, double* time, void* stream, int* status_out) {
+  if (!h) return fail(PR_ERR_ARG, "NULL handle");
+  unsigned char* fx = noalloc_scratch(h->device);
+  if (!fx) return fail(PR_ERR_HIP, "the no-allocation scratch of this device was not resolved by pr_create");
+  return react_box_core(h, lo, hi, rY, rYsrc, T, rH, rHsrc, FC, mask, dt_react, time, stream, status_out, fx,
+                        kNoAllocScratchBytes);
 }
 
 int pr_react_batch_enqueue(pr_handle* h, int nbox, const pr_box* boxes, void* stream) {

@@ -22,7 +22,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpelelmex_amd.so")
 _LIB = None
 
-EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_react_batch", "pr_react_batch_enqueue", "pr_collect", "pr_rhs_batch", "pr_jac_batch",
+EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_react_box_noalloc", "pr_react_batch", "pr_react_batch_enqueue", "pr_collect", "pr_rhs_batch", "pr_jac_batch",
            "pr_reaction_rate_box", "pr_box_costs", "pr_set_temperature_box", "pr_floor_species_box",
            "pr_heat_release_box", "pr_reaction_source_box", "pr_advance_chemistry_box",
            "pr_advance_chemistry_chem_box", "pr_set_eos_parm", "pr_surrogate_load", "pr_surrogate_features", "pr_surrogate_forward", "pr_mlp_gemm",
@@ -75,6 +75,8 @@ def lib():
                                    C.POINTER(Array4), C.POINTER(IArray4), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
         L.pr_react_box.restype = C.c_int
+        L.pr_react_box_noalloc.argtypes = L.pr_react_box.argtypes
+        L.pr_react_box_noalloc.restype = C.c_int
         L.pr_react_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(Box), C.c_void_p]
         L.pr_react_batch.restype = C.c_int
         L.pr_react_batch_enqueue.argtypes = [C.c_void_p, C.c_int, C.POINTER(Box), C.c_void_p]
@@ -173,8 +175,10 @@ class Reactor:
         if rc != 0:
             raise ReactorError(lib().pr_last_error().decode())
 
-    def react(self, rY, rYsrc, T, rH, rHsrc, FC, mask, dt_react, time=0.0, stream=None, status=None):
-        """Box react on device tensors; returns (n_failed, dt_react, time)."""
+    def react(self, rY, rYsrc, T, rH, rHsrc, FC, mask, dt_react, time=0.0, stream=None, status=None,
+              noalloc=False):
+        """Box react on device tensors; returns (n_failed, dt_react, time). noalloc=True calls
+        pr_react_box_noalloc (no device allocation; the library's static scratch)."""
         import torch
         if self._h is None:
             raise ReactorError("react() before init()")
@@ -189,11 +193,12 @@ class Reactor:
         dt = C.c_double(dt_react)
         tm = C.c_double(time)
         st = stream if stream is not None else torch.cuda.current_stream()
-        rc = lib().pr_react_box(self._h, lo, hi, *[C.byref(x) for x in a], C.byref(m) if m is not None else None,
-                                C.byref(dt), C.byref(tm), C.c_void_p(st.cuda_stream),
-                                C.c_void_p(status.data_ptr()) if status is not None else None)
+        fn = lib().pr_react_box_noalloc if noalloc else lib().pr_react_box
+        rc = fn(self._h, lo, hi, *[C.byref(x) for x in a], C.byref(m) if m is not None else None,
+                C.byref(dt), C.byref(tm), C.c_void_p(st.cuda_stream),
+                C.c_void_p(status.data_ptr()) if status is not None else None)
         if rc < 0:
-            raise ReactorError(f"pr_react_box failed ({rc}): {lib().pr_last_error().decode()}")
+            raise ReactorError(f"{fn.__name__} failed ({rc}): {lib().pr_last_error().decode()}")
         return rc, dt.value, tm.value
 
     def react_batch(self, boxes, stream=None, enqueue_only=False):
