@@ -119,7 +119,7 @@ def gpu_vs_oracle(gpu_state, ref):
     the CPU oracle's result on the cells it integrated: bit-exactness and the worst errors."""
     idx, out = ref
     N = out["rY"].shape[0]
-    gY = gpu_state["rY"].reshape(N, -1).cpu().numpy()[:, idx]
+    gY = np.ascontiguousarray(gpu_state["rY"].reshape(N, -1).cpu().numpy()[:, idx])  # summed like the oracle's
     gT = gpu_state["T"].reshape(-1).cpu().numpy()[idx]
     gF = gpu_state["FC"].reshape(-1).cpu().numpy()[idx]
     act = out["status"] == 0

@@ -30,6 +30,8 @@
  *                    PeleLM::computeInstantaneousReactionRate runs it per box
  *                    (Source/PeleLMeX_Reactions.cpp:399-435)
  *   pr_box_costs     PeleLM::computeCosts for the chemistry BoxArray (Source/PeleLMeX_Regrid.cpp:558-606)
+ *   pr_chem_boxes    the chemistry BoxArray and its flags built in PeleLM::resetCoveredMask
+ *                    (Source/PeleLMeX_Utils.cpp:728-788)
  *   pr_set_temperature_box  PeleLM::setTemperature per box (Source/PeleLMeX_Eos.cpp:218-240,
  *                    getTfromHY Source/PeleLMeX_K.H:700-723)
  *   pr_floor_species_box    PeleLM::floorSpecies per box (Source/PeleLMeX_Utils.cpp:649-690)
@@ -179,6 +181,18 @@ int pr_reaction_rate_box(pr_handle* h, const int lo[3], const int hi[3], const p
    host array of nbox doubles. Waits for `stream`. */
 int pr_box_costs(pr_handle* h, int nbox, const int* lo, const int* hi, const pr_array4* FC, int method,
                  double* costs, void* stream);
+/* The chemistry BoxArray of a level and its flags, as PeleLM::resetCoveredMask builds them
+   (Source/PeleLMeX_Utils.cpp:728-788; host only, no handle). Below the finest level: the cells
+   of the level's grids that the finer level does not cover (domain minus the finer grids, given
+   at the finer resolution and coarsened by ref_ratio, intersected with the grids), chopped so no
+   box exceeds max_grid_size_chem in any direction when all three entries are > 0, flag 1
+   (reacted); then the coarsened finer grids, flag 0 (advanceChemistryBAChem gives them FC = 0
+   without reacting them). Finest level (finest != 0): the grids, chopped the same way, all flag 1.
+   Boxes are cell-centred, lo / hi inclusive, 3 ints per box. Returns the number of chemistry
+   boxes n and writes the first min(n, cap) (call again with cap >= n), or a negative error. */
+int pr_chem_boxes(const int domain_lo[3], const int domain_hi[3], int ngrid, const int* grid_lo, const int* grid_hi,
+                  int nfine, const int* fine_lo, const int* fine_hi, int ref_ratio, const int max_grid_size_chem[3],
+                  int finest, int cap, int* box_lo, int* box_hi, int* flags);
 /* Equation-of-state box kernels around the chemistry, MKS Array4 views as PeleLMeX's state
    (rho kg m^-3, rY kg m^-3, rH J m^-3, T K), one enqueue on `stream` then a wait for it.
    pr_set_temperature_box (setTemperature, Source/PeleLMeX_Eos.cpp:218-240 with getTfromHY,
@@ -249,7 +263,9 @@ int pr_set_eos_parm(pr_handle* h, const void* eos_parm, const char* eos_model);
    evaluates it in BF16 on the matrix cores, accepts the cells that pass the FP64 trust check
    (element balance, backward-Euler residual with the analytic rates at surrogate.trust_rtol /
    surrogate.trust_atol) and integrates the others with the BDF; surrogate.audit_frac of the
-   accepted cells are re-solved by the BDF and their worst error reported in pr_stats. */
+   accepted cells are re-solved by the BDF and their worst error reported in pr_stats. With
+   surrogate.zero_increment = 1 the same path runs with every increment zero (the exact source
+   step alone, no network pass): the baseline a trained network is measured against. */
 int pr_surrogate_load(pr_handle* h, int nlayers, const int* dims, const float* params, const float* xnorm,
                       const float* ynorm);
 /* The loaded network on given inputs (no reference counterpart): X device bf16 [ncells x dims[0]]

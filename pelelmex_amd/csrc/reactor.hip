@@ -19,6 +19,7 @@
 
 #include "../../include/pele_reactor_capi.h"
 #include "surrogate_kernel.h"
+#include "chem_boxes.h"
 #include "eos_kernel.h"
 #include "mech/drm19.h"
 #include "mech/lidryer.h"
@@ -263,6 +264,9 @@ struct Keys {
   int surr_enable = 1;
   double surr_trust_rtol = 1e-3, surr_trust_atol = 1e-9, surr_elem_tol = 0.05, surr_audit_frac = 0.01;
   int surr_audit_seed = 12345;
+  // 1: the learned path with every increment zero (the source step alone, no network pass): the
+  // predictor the network has to beat, under the same trust check, fallback and audit
+  int surr_zero = 0;
 };
 
 int parse_keys(const char* text, Keys& K) {
@@ -352,6 +356,9 @@ int parse_keys(const char* text, Keys& K) {
     } else if (k == "surrogate.audit_frac") {
       if (!num(d) || d < 0 || d > 1) return fail(PR_ERR_KEY, "surrogate.audit_frac must be in [0, 1]");
       K.surr_audit_frac = d;
+    } else if (k == "surrogate.zero_increment") {
+      if (!num(d) || (d != 0 && d != 1)) return fail(PR_ERR_KEY, "surrogate.zero_increment must be 0 or 1");
+      K.surr_zero = (int)d;
     } else if (k == "surrogate.audit_seed") {
       if (!num(d)) return fail(PR_ERR_KEY, "bad surrogate.audit_seed");
       K.surr_audit_seed = (int)d;
@@ -986,6 +993,7 @@ static int surr_trust(pr_handle* h, const prk::BoxDesc& d, long base, hipStream_
   Q.aud_dt = S->aud_dt;
   Q.Tstar = S->Tstar + base;
   Q.scale = S->scale + base * (N + 1);
+  Q.zero = h->keys.surr_zero;
   HIPCHK(h->var->trust(Q, gblocks, block, shmem, st));
   return 0;
 }
@@ -1209,7 +1217,7 @@ static int react_box_core(pr_handle* h, const int lo[3], const int hi[3], const 
     int rc = ensure_surr_buffers(h, ncells);
     if (rc) return rc;
     if ((rc = surr_prep(h, P.one, 0, st))) return rc;
-    if ((rc = mlp_forward(h, ncells, st))) return rc;
+    if (!h->keys.surr_zero && (rc = mlp_forward(h, ncells, st))) return rc;
     HIPCHK(hipMemsetAsync(S->cnt, 0, 8 * sizeof(unsigned int), st));
     if ((rc = surr_trust(h, P.one, 0, st))) return rc;
     P.cell_list = S->rej;
@@ -1404,7 +1412,7 @@ int pr_react_batch_enqueue(pr_handle* h, int nbox, const pr_box* boxes, void* st
     if (rc) return rc;
     for (int b = 0; b < nbox; ++b)
       if ((rc = surr_prep(h, descs[(size_t)b], starts[(size_t)b], st))) return rc;
-    if ((rc = mlp_forward(h, total, st))) return rc;
+    if (!h->keys.surr_zero && (rc = mlp_forward(h, total, st))) return rc;
     hipLaunchKernelGGL(prk::zero_u32, dim3(1), dim3(64), 0, st, S->cnt, 8);
     HIPCHK(hipGetLastError());
     for (int b = 0; b < nbox; ++b)
@@ -1616,6 +1624,47 @@ static int eos_launch(pr_handle* h, int mode, const int lo[3], const int hi[3], 
   HIPCHK(h->var->eos(mode, P, blocks, block, shmem, st));
   HIPCHK(hipStreamSynchronize(st));
   return 0;
+}
+
+int pr_chem_boxes(const int domain_lo[3], const int domain_hi[3], int ngrid, const int* grid_lo, const int* grid_hi,
+                  int nfine, const int* fine_lo, const int* fine_hi, int ref_ratio, const int max_grid_size_chem[3],
+                  int finest, int cap, int* box_lo, int* box_hi, int* flags) {
+  if (!domain_lo || !domain_hi || !max_grid_size_chem || ngrid < 0 || nfine < 0 || cap < 0)
+    return fail(PR_ERR_ARG, "bad argument");
+  if ((ngrid > 0 && (!grid_lo || !grid_hi)) || (nfine > 0 && (!fine_lo || !fine_hi)) ||
+      (cap > 0 && (!box_lo || !box_hi || !flags)))
+    return fail(PR_ERR_ARG, "NULL array");
+  if (!finest && ref_ratio < 1) return fail(PR_ERR_ARG, "ref_ratio must be >= 1 below the finest level");
+  auto box = [](const int* lo, const int* hi) {
+    prc::IBox b;
+    for (int d = 0; d < 3; ++d) {
+      b.lo[d] = lo[d];
+      b.hi[d] = hi[d];
+    }
+    return b;
+  };
+  const prc::IBox dom = box(domain_lo, domain_hi);
+  if (!dom.ok()) return fail(PR_ERR_ARG, "empty domain");
+  std::vector<prc::IBox> grids, fine, out;
+  std::vector<int> fl;
+  for (int g = 0; g < ngrid; ++g) {
+    grids.push_back(box(grid_lo + 3 * g, grid_hi + 3 * g));
+    if (!grids.back().ok()) return fail(PR_ERR_ARG, "empty grid box");
+  }
+  for (int g = 0; g < nfine; ++g) {
+    fine.push_back(box(fine_lo + 3 * g, fine_hi + 3 * g));
+    if (!fine.back().ok()) return fail(PR_ERR_ARG, "empty finer-level box");
+  }
+  prc::chem_box_array(dom, grids, fine, ref_ratio, max_grid_size_chem, finest != 0, out, fl);
+  if (out.size() > 0x7fffffffu) return fail(PR_ERR_ARG, "too many boxes");
+  for (size_t b = 0; b < out.size() && (long)b < cap; ++b) {
+    for (int d = 0; d < 3; ++d) {
+      box_lo[3 * b + d] = out[b].lo[d];
+      box_hi[3 * b + d] = out[b].hi[d];
+    }
+    flags[b] = fl[b];
+  }
+  return (int)out.size();
 }
 
 int pr_box_costs(pr_handle* h, int nbox, const int* lo, const int* hi, const pr_array4* FC, int method,

@@ -197,6 +197,7 @@ struct TrustParams {
   const float* yeps;    // rate-scale floors (0: the output is the increment itself)
   const double* Tstar;  // T after the source step (prep kernel)
   const double* scale;  // [ncells x (N + 1)] rate scales dt |f_i(y*)| (/ rho* for species)
+  int zero;             // 1: every increment is zero (the source step alone; Yout is not read)
   const double* Mel;  // [E x N] elements per gram of species k
   const double* Minv; // [E x E] (M M^T)^-1
   int E;
@@ -279,13 +280,13 @@ __global__ void __launch_bounds__(BLOCK) trust_kernel(TrustParams P) {
           y0[s] = 0.0;
         }
         src[s] = P.rYsrc.p[offS + (long)ii * P.rYsrc.ns];
-        dY[s] = decode(o[ii], P.ymu[ii], P.ysd[ii], P.yeps[ii], P.scale[c * (long)n + ii]);
+        dY[s] = P.zero ? 0.0 : decode(o[ii], P.ymu[ii], P.ysd[ii], P.yeps[ii], P.scale[c * (long)n + ii]);
       }
     }
     const double T0 = P.T.p[aoff(P.T, i, j, k)];
     const double rhoh0 = P.rH.p[aoff(P.rH, i, j, k)];
     const double srch = P.rHsrc.p[aoff(P.rHsrc, i, j, k)];
-    const double dT = decode(o[N], P.ymu[N], P.ysd[N], P.yeps[N], P.scale[c * (long)n + N]);
+    const double dT = P.zero ? 0.0 : decode(o[N], P.ymu[N], P.ysd[N], P.yeps[N], P.scale[c * (long)n + N]);
     // element balance of the increments and its least-squares removal
     double a[SURR_MAXE], gross[SURR_MAXE];
 #pragma unroll

@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(_HERE, "libpelelmex_amd.so")
 _LIB = None
 
 EXPORTS = ["pr_create", "pr_set_typ_vals", "pr_react_box", "pr_react_box_noalloc", "pr_react_batch", "pr_react_batch_enqueue", "pr_collect", "pr_rhs_batch", "pr_jac_batch",
-           "pr_reaction_rate_box", "pr_box_costs", "pr_set_temperature_box", "pr_floor_species_box",
+           "pr_reaction_rate_box", "pr_box_costs", "pr_chem_boxes", "pr_set_temperature_box", "pr_floor_species_box",
            "pr_heat_release_box", "pr_reaction_source_box", "pr_advance_chemistry_box",
            "pr_advance_chemistry_chem_box", "pr_set_eos_parm", "pr_surrogate_load", "pr_surrogate_features", "pr_surrogate_forward", "pr_mlp_gemm",
            "pr_mlp_gelu_backward", "pr_mlp_column_sum", "pr_mlp_mse", "pr_mlp_adam", "pr_mlp_to_bf16",
@@ -115,6 +115,10 @@ def lib():
         L.pr_mlp_mse.argtypes = [i64, C.c_int, vp, i64, vp, i64, vp, vp, vp]
         L.pr_mlp_adam.argtypes = [i64, vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, vp]
         L.pr_mlp_to_bf16.argtypes = [i64, vp, vp, vp]
+        ip3 = C.POINTER(C.c_int)
+        L.pr_chem_boxes.argtypes = [ip3, ip3, C.c_int, ip3, ip3, C.c_int, ip3, ip3, C.c_int, ip3, C.c_int, C.c_int,
+                                    ip3, ip3, ip3]
+        L.pr_chem_boxes.restype = C.c_int
         L.pr_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.pr_num_species.argtypes = [C.c_void_p]
         L.pr_destroy.argtypes = [C.c_void_p]
@@ -448,6 +452,68 @@ class Reactor:
             self.close()
         except Exception:
             pass
+
+
+def chem_boxes(domain, grids, fine=(), ref_ratio=2, max_grid_size_chem=(0, 0, 0), finest=False):
+    """The chemistry BoxArray of a level (pr_chem_boxes; PeleLM::resetCoveredMask,
+    Source/PeleLMeX_Utils.cpp:728-788). domain: (lo, hi); grids / fine: lists of (lo, hi) with
+    inclusive cell-index triples, fine at the finer level's resolution. Returns (boxes, flags):
+    the uncovered (reacted, flag 1) boxes chopped to max_grid_size_chem, then the coarsened
+    finer-level boxes (flag 0); on the finest level its grids chopped, all flag 1."""
+    import numpy as np
+    def arr(bs, k):
+        a = np.ascontiguousarray(np.array([b[k] for b in bs], dtype=np.int32).reshape(-1, 3))
+        return a
+    dlo = (C.c_int * 3)(*domain[0])
+    dhi = (C.c_int * 3)(*domain[1])
+    glo, ghi = arr(grids, 0), arr(grids, 1)
+    flo, fhi = arr(fine, 0), arr(fine, 1)
+    ms = (C.c_int * 3)(*max_grid_size_chem)
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    cap = 0
+    while True:
+        blo = np.zeros((max(cap, 1), 3), dtype=np.int32)
+        bhi = np.zeros((max(cap, 1), 3), dtype=np.int32)
+        fl = np.zeros(max(cap, 1), dtype=np.int32)
+        n = lib().pr_chem_boxes(dlo, dhi, len(grids), ip(glo), ip(ghi), len(fine), ip(flo), ip(fhi), int(ref_ratio), ms,
+                                int(bool(finest)), cap, ip(blo), ip(bhi), ip(fl))
+        if n < 0:
+            raise ReactorError(f"pr_chem_boxes failed ({n}): {lib().pr_last_error().decode()}")
+        if n <= cap:
+            return [(tuple(int(x) for x in blo[b]), tuple(int(x) for x in bhi[b])) for b in range(n)], \
+                [int(x) for x in fl[:n]]
+        cap = n
+
+
+def advance_chemistry_bachem(reactor, domain_lo, old, new, F_rY, F_rH, FC, IR, dt, boxes, flags, mask=None,
+                             stream=None):
+    """PeleLM::advanceChemistryBAChem for one level (Source/PeleLMeX_Reactions.cpp:180-367) on
+    device tensors laid out over the level's index space from domain_lo (MKS): old / new dicts of
+    rY [N, nz, ny, nx], rH, T [nz, ny, nx]; the forcing F_rY, F_rH; FC and I_R written. Each
+    chemistry box (boxes, flags from chem_boxes) gets a contiguous copy of the old state and of
+    the forcing (ParallelCopy, :207-208), the per-box body pr_advance_chemistry_chem_box (react
+    when flagged, else FC = 0), and its state and FC copied back into new / FC (:311-312); I_R =
+    -(rY_old - rY_new) dt_inv - F_rY over the boxes (:322-347). Returns the failed-cell count."""
+    import torch
+    nfail = 0
+    dt_inv = 1.0 / float(dt)
+    for (lo, hi), flag in zip(boxes, flags):
+        sl = tuple(slice(lo[d] - domain_lo[d], hi[d] - domain_lo[d] + 1) for d in (2, 1, 0))
+        cY = old["rY"][(slice(None),) + sl].contiguous()
+        cH = old["rH"][sl].contiguous()
+        cT = old["T"][sl].contiguous()
+        fY = F_rY[(slice(None),) + sl].contiguous()
+        fH = F_rH[sl].contiguous()
+        cF = torch.empty_like(cT)
+        m = mask[sl].contiguous() if mask is not None else None
+        nfail += reactor.advance_chemistry_chem(cY, cH, cT, fY, fH, cF, m, dt, do_reaction=bool(flag), stream=stream)
+        new["rY"][(slice(None),) + sl] = cY
+        new["rH"][sl] = cH
+        new["T"][sl] = cT
+        FC[sl] = cF
+        oY = old["rY"][(slice(None),) + sl]
+        IR[(slice(None),) + sl] = -(oY - cY) * dt_inv - F_rY[(slice(None),) + sl]
+    return nfail
 
 
 def typical_values_chem(rY_mks, T, Ymin=1.0e-10):

@@ -610,6 +610,52 @@ def test_advance_chemistry_chem_box_bits(gpu, do_reaction):
         np.testing.assert_array_equal(host(tY, N), (rY * 1.0e-3) * 1.0e3)
 
 
+def test_advance_chemistry_bachem_level(gpu):
+    """advanceChemistryBAChem over a whole level (Source/PeleLMeX_Reactions.cpp:180-367): the
+    chemistry BoxArray from pr_chem_boxes (two grids, a finer level covering part of them,
+    max_grid_size_chem 4), the state and forcing copied into each chemistry box, the per-box
+    body (react on uncovered boxes, FC = 0 on covered ones), the copies back and I_R - every
+    cell bit for bit with the oracle's per-cell sequence (react on the cells the finer level
+    does not cover)."""
+    import torch
+    from oracle import pyoracle as po
+    from pelelmex_amd.reactor import Reactor, advance_chemistry_bachem, chem_boxes
+    nx, ny, nz = 16, 8, 4
+    b = W.ch4_air_flame_box(nx=nx, ny=ny, nz=nz, seed=65, dt=2e-6)
+    N, n = b["rY"].shape
+    dom = ((0, 0, 0), (nx - 1, ny - 1, nz - 1))
+    grids = [((0, 0, 0), (7, 7, 3)), ((8, 0, 0), (15, 7, 3))]
+    fine = [((6, 2, 0), (17, 9, 5))]                      # at ref_ratio 2: cells (3..8, 1..4, 0..2)
+    boxes, flags = chem_boxes(dom, grids, fine, ref_ratio=2, max_grid_size_chem=(4, 4, 4))
+    assert flags.count(0) == 1 and flags.count(1) > 4
+    ii, jj, kk = np.arange(n) % nx, (np.arange(n) // nx) % ny, np.arange(n) // (nx * ny)
+    covered = (ii >= 3) & (ii <= 8) & (jj >= 1) & (jj <= 4) & (kk <= 2)
+    rY, rH, T = b["rY"] * 1.0e3, b["rH"] * 0.1, b["T"].copy()
+    F_rY, F_rH = b["rYsrc"] * 1.0e3, b["rHsrc"] * 0.1
+    m = po.Mech("drm19")
+    react = po.advance_chemistry_chem(m, rY, rH, T, F_rY, F_rH, b["mask"], b["dt"], do_reaction=True, max_order=4)
+    skip = po.advance_chemistry_chem(m, rY, rH, T, F_rY, F_rH, b["mask"], b["dt"], do_reaction=False, max_order=4)
+    ref = {k: np.where(covered, skip[k], react[k]) for k in ("rY", "rH", "T", "FC")}
+    ref["IR"] = -(rY - ref["rY"]) * (1.0 / b["dt"]) - F_rY
+    dev = lambda a, c: torch.as_tensor(np.ascontiguousarray(a)).reshape((c, nz, ny, nx) if c else (nz, ny, nx)).to(gpu)
+    old = {"rY": dev(rY, N), "rH": dev(rH, 0), "T": dev(T, 0)}
+    new = {k: torch.full_like(v, -1.0) for k, v in old.items()}
+    FY, FH = dev(F_rY, N), dev(F_rH, 0)
+    FC = torch.full_like(old["T"], 7.0)
+    IR = torch.zeros_like(old["rY"])
+    r = Reactor.create("ReactorCvode", "drm19", dict(KEYS, **{"cvode.max_order": 4}))
+    r.init(2, 1)
+    assert advance_chemistry_bachem(r, dom[0], old, new, FY, FH, FC, IR, b["dt"], boxes, flags) == 0
+    torch.cuda.synchronize()
+    r.close()
+    host = lambda t, c: t.cpu().numpy().reshape((c, n) if c else (n,))
+    for name, got, c in (("rY", new["rY"], N), ("rH", new["rH"], 0), ("T", new["T"], 0), ("FC", FC, 0),
+                         ("IR", IR, N)):
+        _assert_bits(name, host(got, c), ref[name])
+    assert np.all(host(FC, 0)[covered] == 0) and np.all(host(FC, 0)[~covered] > 0)
+    np.testing.assert_array_equal(host(FY, N), F_rY)   # the caller's forcing is not touched
+
+
 def test_one_handle_from_two_host_threads(gpu):
     """Calls on one handle from two host threads at once (as PeleLMeX's OpenMP box loop would
     make them) are serialized by the library: each thread's boxes get the bits a sequential
