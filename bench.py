@@ -237,6 +237,19 @@ def main():
         surrogate.attach(rs, model)
         ms_sur, sstats, _, sur_state = measure(rs)
         rs.close()
+        # the same path with every increment zero (source step alone, no network pass): the
+        # predictor the network is measured against, under the same trust check and tolerances
+        rz = Reactor.create("ReactorCvode", "drm19", dict(skeys, **{"surrogate.zero_increment": 1}))
+        rz.init(2, 1)
+        surrogate.attach(rz, model)
+        ms_zero, zstats, _, zero_state = measure(rz)
+        rz.close()
+        z2 = zstats[-1]
+        bdf_fc = bdf_state["FC"].reshape(-1).double()
+
+        def work_removed(state):  # share of the BDF's right-hand-side evaluations the accepted cells carry
+            acc = state["FC"].reshape(-1) == 1.0
+            return float(bdf_fc[acc].sum() / bdf_fc.sum())
         s2 = sstats[-1]
         N = bdf_state["rY"].shape[0]
         dT = ((sur_state["T"] - bdf_state["T"]).abs() / bdf_state["T"]).max().item()
@@ -251,7 +264,14 @@ def main():
                "max_dT_rel_vs_bdf_all_cells": dT,
                "max_dY_vs_bdf_all_cells": (Ys - Yb).abs().max().item(),
                "speedup_vs_bdf": bdf_value and (ncells / (ms_sur / 1e3)) / bdf_value,
-               "weights": os.path.relpath(wpath, ROOT)}
+               "bdf_work_removed": work_removed(sur_state),
+               "weights": os.path.relpath(wpath, ROOT),
+               "zero_increment_predictor": {
+                   "value": ncells / (ms_zero / 1e3), "ms_per_step": ms_zero,
+                   "accepted": z2["sur_accepted"] / ncells, "bdf_work_removed": work_removed(zero_state),
+                   "max_dT_rel_vs_bdf_all_cells":
+                       ((zero_state["T"] - bdf_state["T"]).abs() / bdf_state["T"]).max().item(),
+                   "speedup_vs_bdf": (ncells / (ms_zero / 1e3)) / bdf_value}}
 
     value = sur["value"] if sur else bdf_value
     line = {
@@ -395,7 +415,7 @@ def bench_amr(args, mech="drm19"):
 
     def timed(with_model):
         skeys = dict(keys)
-        if with_model:
+        if with_model and "trust_rtol" in model:  # a network whose calibration found no tolerance keeps the defaults
             skeys["surrogate.trust_rtol"] = float(model["trust_rtol"])
             skeys["surrogate.trust_atol"] = float(model["trust_atol"])
         r = Reactor.create("ReactorCvode", mech, skeys)
@@ -577,8 +597,10 @@ def bench_chunks(args, mech="dodecane_lu"):
                        "newton_per_cell": tot["nni"] / tot["cells"], "err_fail_per_cell": tot["netf"] / tot["cells"]},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(host0, min(args.cpu_sample, 65536), max_order=args.max_order,
-                                            what="the first 64^3 chunk (masked cells count as cells)")[0]
+        line["cpu_baseline"], ref = cpu_baseline(host0, min(args.cpu_sample, 65536), max_order=args.max_order,
+                                                 what="the first 64^3 chunk (masked cells count as cells)")
+        line["parity"] = gpu_vs_oracle(chunks[0][1], ref)   # chunk 0's tensors hold its last result
+        line["parity"]["box"] = "the first 64^3 chunk, the CPU sample's cells"
     print(json.dumps(line), flush=True)
 
 
