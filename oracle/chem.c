@@ -328,7 +328,7 @@ double orc_exp(double x) { return prm_exp(x); }
 double orc_log(double x) { return prm_log(x); }
 
 #define HY2T_MAXIT 20
-#define HY2T_RTOL 1e-14
+#define HY2T_RTOL 1e-10
 
 /* Newton on T for h(T, Y) = h; Y given as rhoY and rho (Y_k = rhoY_k / rho). */
 static int hy2t_rhoY(int id, double h, const double* rY, double rho, double* T) {

@@ -186,19 +186,19 @@ def generate_pairs(reactor, seeds, dts, nx=64, device="cuda", log=print, mech="d
         rY0, T0 = t["rY"].reshape(reactor.nsp, -1).clone(), t["T"].reshape(-1).clone()
         rH0 = t["rH"].reshape(-1).clone()
         t0 = time.perf_counter()
-        nfail, _, _ = reactor.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], None, dt, 0.0)
+        status = torch.zeros(T0.numel(), dtype=torch.int32, device=device)
+        nfail, _, _ = reactor.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], None, dt, 0.0,
+                                    status=status)
         torch.cuda.synchronize()
         log(f"  pairs: seed {seed} dt {dt:.2e} cells {T0.numel()} BDF {time.perf_counter() - t0:.2f} s, failed {nfail}")
-        ok = torch.ones(T0.numel(), dtype=torch.bool, device=device) if nfail == 0 else None
-        if ok is None:
-            raise ReactorError(f"{nfail} cells failed while generating training pairs")
-        out["rY0"].append(rY0)
-        out["T0"].append(T0)
-        out["rH0"].append(rH0)
-        out["dt"].append(torch.full_like(T0, dt))
-        out["rY1"].append(t["rY"].reshape(reactor.nsp, -1).clone())
-        out["T1"].append(t["T"].reshape(-1).clone())
-        out["nfe"].append(t["FC"].reshape(-1).clone())
+        ok = status == 0   # a cell whose integration failed gives no pair
+        out["rY0"].append(rY0[:, ok])
+        out["T0"].append(T0[ok])
+        out["rH0"].append(rH0[ok])
+        out["dt"].append(torch.full_like(T0[ok], dt))
+        out["rY1"].append(t["rY"].reshape(reactor.nsp, -1)[:, ok].clone())
+        out["T1"].append(t["T"].reshape(-1)[ok].clone())
+        out["nfe"].append(t["FC"].reshape(-1)[ok].clone())
     return {"rY0": torch.cat(out["rY0"], 1), "T0": torch.cat(out["T0"]), "rH0": torch.cat(out["rH0"]),
             "dt": torch.cat(out["dt"]),
             "rY1": torch.cat(out["rY1"], 1), "T1": torch.cat(out["T1"]), "nfe": torch.cat(out["nfe"])}
