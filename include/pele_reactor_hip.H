@@ -136,6 +136,21 @@ class ReactorHipBDF {
     return rc;
   }
 
+  // react for callers that cannot allocate device memory (no reference counterpart;
+  // pr_react_box_noalloc: the library's static per-GPU scratch, its limits in the C header)
+  template <class Box, class A4, class I4>
+  int react_noalloc(const Box& box, A4 const& rY, A4 const& rYsrc, A4 const& T, A4 const& rE, A4 const& rEsrc,
+                    A4 const& FC, I4 const& mask, double& dt_react, double& time, void* stream = nullptr) {
+    const int lo[3] = {box.smallEnd(0), box.smallEnd(1), box.smallEnd(2)};
+    const int hi[3] = {box.bigEnd(0), box.bigEnd(1), box.bigEnd(2)};
+    pr_array4 a[6] = {to_pr(rY), to_pr(rYsrc), to_pr(T), to_pr(rE), to_pr(rEsrc), to_pr(FC)};
+    pr_iarray4 m = to_pr_mask(mask);
+    int rc = pr_react_box_noalloc(h_, lo, hi, &a[0], &a[1], &a[2], &a[3], &a[4], &a[5], &m, &dt_react, &time,
+                                  stream, nullptr);
+    if (rc < 0) throw ReactorError(std::string("pr_react_box_noalloc: ") + pr_last_error());
+    return rc;
+  }
+
   void close() {
     if (h_) pr_destroy(h_);
     h_ = nullptr;
@@ -151,6 +166,31 @@ class ReactorHipBDF {
   std::string integrator_, mechanism_, keys_;
   pr_handle* h_ = nullptr;
 };
+
+// The chemistry BoxArray of a level and its flags (PeleLM::resetCoveredMask,
+// Source/PeleLMeX_Utils.cpp:728-788) through pr_chem_boxes: lo / hi triples of the level's grids
+// and of the finer level's grids (at their resolution; empty on the finest level). Returns the
+// chemistry boxes' lo / hi triples and flags (1: react, 0: covered by the finer level).
+struct ChemBoxes {
+  std::vector<int> lo, hi, flag;
+};
+inline ChemBoxes chem_boxes(const int domain_lo[3], const int domain_hi[3], const std::vector<int>& grid_lo,
+                            const std::vector<int>& grid_hi, const std::vector<int>& fine_lo,
+                            const std::vector<int>& fine_hi, int ref_ratio, const int max_grid_size_chem[3],
+                            bool finest) {
+  const int ng = static_cast<int>(grid_lo.size() / 3), nf = static_cast<int>(fine_lo.size() / 3);
+  ChemBoxes out;
+  int n = pr_chem_boxes(domain_lo, domain_hi, ng, grid_lo.data(), grid_hi.data(), nf, fine_lo.data(),
+                        fine_hi.data(), ref_ratio, max_grid_size_chem, finest ? 1 : 0, 0, nullptr, nullptr, nullptr);
+  if (n < 0) throw ReactorError(std::string("pr_chem_boxes: ") + pr_last_error());
+  out.lo.resize(3 * static_cast<size_t>(n));
+  out.hi.resize(3 * static_cast<size_t>(n));
+  out.flag.resize(static_cast<size_t>(n));
+  n = pr_chem_boxes(domain_lo, domain_hi, ng, grid_lo.data(), grid_hi.data(), nf, fine_lo.data(), fine_hi.data(),
+                    ref_ratio, max_grid_size_chem, finest ? 1 : 0, n, out.lo.data(), out.hi.data(), out.flag.data());
+  if (n < 0) throw ReactorError(std::string("pr_chem_boxes: ") + pr_last_error());
+  return out;
+}
 
 }  // namespace pele_hip
 #endif

@@ -112,11 +112,18 @@ struct IntVect { int v[3]; int operator[](int i) const { return v[i]; } };
 struct Box { IntVect lo, hi; int smallEnd(int d) const { return lo[d]; } int bigEnd(int d) const { return hi[d]; } };
 template <class T> struct Array4 { T* p; long long jstride, kstride, nstride; Dim3 begin, end; int ncomp; };
 int main() {
+  const int dlo[3] = {0, 0, 0}, dhi[3] = {15, 15, 15}, mgs[3] = {8, 8, 8};
+  pele_hip::ChemBoxes cb = pele_hip::chem_boxes(dlo, dhi, {0, 0, 0}, {15, 15, 15}, {8, 8, 8}, {15, 15, 15}, 2, mgs,
+                                                false);
+  if (cb.flag.size() < 2 || cb.flag.back() != 0 || cb.flag.front() != 1) return 2;
   pele_hip::ReactorHipBDF r("ReactorCvode", "drm19", "ode.rtol = -5");
   try { r.init(2, 1); } catch (const pele_hip::ReactorError& e) { std::printf("%s\n", e.what()); return 0; }
   return 1;
 }
 template int pele_hip::ReactorHipBDF::react<Box, Array4<double>, Array4<int>>(
+    const Box&, Array4<double> const&, Array4<double> const&, Array4<double> const&, Array4<double> const&,
+    Array4<double> const&, Array4<double> const&, Array4<int> const&, double&, double&, void*);
+template int pele_hip::ReactorHipBDF::react_noalloc<Box, Array4<double>, Array4<int>>(
     const Box&, Array4<double> const&, Array4<double> const&, Array4<double> const&, Array4<double> const&,
     Array4<double> const&, Array4<double> const&, Array4<int> const&, double&, double&, void*);
 struct EosParmStandIn { int dummy; };
