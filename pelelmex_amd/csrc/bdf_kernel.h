@@ -210,14 +210,23 @@ __device__ __forceinline__ double bperm_d(double v, int addr) {
 __device__ __forceinline__ bool upper_half() { return (threadIdx.x & 32) != 0; }
 __device__ __forceinline__ double hsel(double lo, double hi) { return upper_half() ? hi : lo; }
 
+// max of two values that are never NaN, as one v_max_f64: fmax would first canonicalize an
+// operand the compiler cannot prove quiet (one more instruction per step of every reduction)
+__device__ __forceinline__ double max_nn(double a, double b) {
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
 // butterfly over the 32 lanes of each half: DPP within rows, permlane16 swap across rows
+// (MAX: the values must not be NaN)
 template <bool MAX>
 __device__ __forceinline__ double half_reduce(double v) {
-  v = MAX ? fmax(v, dpp_d<0xB1>(v)) : v + dpp_d<0xB1>(v);
-  v = MAX ? fmax(v, dpp_d<0x4E>(v)) : v + dpp_d<0x4E>(v);
-  v = MAX ? fmax(v, dpp_d<0x141>(v)) : v + dpp_d<0x141>(v);
-  v = MAX ? fmax(v, dpp_d<0x140>(v)) : v + dpp_d<0x140>(v);
-  return MAX ? fmax(v, xor16_d(v)) : v + xor16_d(v);
+  v = MAX ? max_nn(v, dpp_d<0xB1>(v)) : v + dpp_d<0xB1>(v);
+  v = MAX ? max_nn(v, dpp_d<0x4E>(v)) : v + dpp_d<0x4E>(v);
+  v = MAX ? max_nn(v, dpp_d<0x141>(v)) : v + dpp_d<0x141>(v);
+  v = MAX ? max_nn(v, dpp_d<0x140>(v)) : v + dpp_d<0x140>(v);
+  return MAX ? max_nn(v, xor16_d(v)) : v + xor16_d(v);
 }
 
 // wave-uniform copy of a value that is identical in all lanes
@@ -251,13 +260,14 @@ __device__ __forceinline__ void gargmax(double& v, int& idx) {
   }
 }
 
+// maximum over the group of values that are never NaN
 template <int G>
 __device__ __forceinline__ double gmax(double v) {
   if constexpr (G == 64) {
-    v = fmax(v, dpp_d<0xB1>(v));
-    v = fmax(v, dpp_d<0x4E>(v));
-    v = fmax(v, dpp_d<0x141>(v));  // quad-uniform values: the mirrored quad is lane ^ 4
-    v = fmax(v, dpp_d<0x140>(v));  // 8-uniform values: the mirrored half row is lane ^ 8
+    v = max_nn(v, dpp_d<0xB1>(v));
+    v = max_nn(v, dpp_d<0x4E>(v));
+    v = max_nn(v, dpp_d<0x141>(v));  // quad-uniform values: the mirrored quad is lane ^ 4
+    v = max_nn(v, dpp_d<0x140>(v));  // 8-uniform values: the mirrored half row is lane ^ 8
     return fmax(fmax(rlane_d(v, 0), rlane_d(v, 16)), fmax(rlane_d(v, 32), rlane_d(v, 48)));
   } else if constexpr (G == 32) {
     return half_reduce<true>(v);

@@ -24,6 +24,7 @@ struct Ctl {
   double tn, h, hscale, hprime, eta, etamax, rl1, gamma, gammap, gamrat, crate, delp, acnrm;
   double saved_tq5, saved_t, t_eval, hlb, hub, hg, rhoh0, rhohsrc;
   double tau[QMAXC + 1], tq[6], l[QMAXC + 1];
+  double rq, rq1, rq2;  // 1 / q, 1 / (q + 1), 1 / (q + 2), formed when q changes (L = q + 1 throughout)
   int q, qprime, L, qwait, nscon, ncf, nef, nflag, convfail, mnewt, jcur, jbad, callSetup, count1;
   int phase, cell, box, nst, nstlp, nstlj, nstloc, nfe, nje, nset, nni, netf, ncfn, guard;
   double tout;
@@ -157,6 +158,15 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
     for (int s = 0; s < RPL; ++s) ewt[s] = 1.0 / (P.rtol * fabs(y[s]) + atol_r[s]);
   };
 
+  // reciprocals of the order the step control divides by, formed once per order change (the same
+  // correctly rounded values the divisions at each use gave)
+  auto set_recips = [&]() __attribute__((always_inline)) {
+    const double q = (double)c.q;
+    c.rq = 1.0 / q;
+    c.rq1 = 1.0 / (q + 1.0);
+    c.rq2 = 1.0 / (q + 2.0);
+  };
+
   auto rescale = [&]() __attribute__((always_inline)) {
     const double eta = c.eta;
     const int q = c.q;
@@ -224,7 +234,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
           for (int i = j; i >= 1; --i) l[i] += l[i - 1] * xi_inv;
         }
       }
-      alpha0 -= 1.0 / q;
+      alpha0 -= c.rq;
       xistar_inv = -l[1] - alpha0;
       double tq1 = 0.0;
 #pragma unroll
@@ -248,7 +258,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
     if (c.qwait == 1) {
       if (q > 1) {
         double C = xistar_inv / lq;
-        double A3 = alpha0 + 1.0 / q;
+        double A3 = alpha0 + c.rq;
         double A4 = alpha0_hat + xi_inv;
         double Cpinv = (1.0 - A4 + A3) / A3;
         c.tq[1] = fabs(C * Cpinv);
@@ -261,7 +271,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
         if (j == q) tqq = tau[j];
       hsum += tqq;
       xi_inv = h / hsum;
-      double A5 = alpha0 - 1.0 / (q + 1);
+      double A5 = alpha0 - c.rq1;
       double A6 = alpha0_hat - xi_inv;
       double Cppinv = (1.0 - A6 + A5) / A2;
       c.tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
@@ -355,6 +365,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
       c.q = c.qprime;
       c.L = c.q + 1;
       c.qwait = c.L;
+      set_recips();
     }
     rescale();
   };
@@ -468,6 +479,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
       c.nfe = c.nje = c.nset = c.nni = c.netf = c.ncfn = 0;
       c.nst = c.nstlp = c.nstlj = 0;
       c.q = 1; c.L = 2; c.qwait = 2; c.qprime = 1;
+      c.rq = 1.0; c.rq1 = 0.5; c.rq2 = 1.0 / 3.0;
       c.etamax = 10000.0; c.nscon = 0; c.saved_tq5 = 0.0; c.crate = 1.0; c.delp = 0.0;
       c.gamrat = 1.0; c.gammap = 1.0; c.tn = 0.0; c.jcur = 0; c.eta = 1.0;
       for (int j2 = 0; j2 <= Q; ++j2) c.tau[j2] = 0.0;
@@ -687,7 +699,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
             c.eta = 1.0;
           } else {
             const int L = c.L;
-            const double etaq = 1.0 / (pr_pow(6.0 * dsm, 1.0 / L) + 1e-6);
+            const double etaq = 1.0 / (pr_pow(6.0 * dsm, c.rq1) + 1e-6);
             if (c.qwait != 0) {
               c.eta = etaq;
               c.qprime = q;
@@ -704,7 +716,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
 #pragma unroll
                     for (int s = 0; s < RPL; ++s) zq[s] = zn[j][s];
                 const double ddn = wrms<N, R, G>(S, lane, zq, ewt) * c.tq[1];
-                etaqm1 = 1.0 / (pr_pow(6.0 * ddn, 1.0 / q) + 1e-6);
+                etaqm1 = 1.0 / (pr_pow(6.0 * ddn, c.rq) + 1e-6);
               }
               if (q != qmax && c.saved_tq5 != 0.0) {
                 const double cquot = (c.tq[5] / c.saved_tq5) * pr_pow(c.h / c.tau[2], (double)L);
@@ -717,7 +729,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
 #pragma unroll
                     for (int s = 0; s < RPL; ++s) tmp[s] = acor[s] - cquot * zn[j][s];
                 const double dup = wrms<N, R, G>(S, lane, tmp, ewt) * c.tq[3];
-                etaqp1 = 1.0 / (pr_pow(10.0 * dup, 1.0 / (L + 1)) + 1e-6);
+                etaqp1 = 1.0 / (pr_pow(10.0 * dup, c.rq2) + 1e-6);
               }
               const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
               if (etam < 1.5) {
@@ -781,7 +793,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
           } else {
             c.etamax = 1.0;
             if (c.nef <= 3) {
-              double eta = 1.0 / (pr_pow(6.0 * dsm, 1.0 / c.L) + 1e-6);
+              double eta = 1.0 / (pr_pow(6.0 * dsm, c.rq1) + 1e-6);
               eta = fmax(0.1, eta);
               if (c.nef >= 2) eta = fmin(eta, 0.2);
               c.eta = eta;
@@ -793,6 +805,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
               c.L = c.q;
               c.q -= 1;
               c.qwait = c.L;
+              set_recips();
               rescale();
               attempt();
             } else {
