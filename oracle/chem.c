@@ -281,11 +281,18 @@ static void state_at(int id, const double* y, double T, state_t* s) {
   s->Ctot = orc_psum(s->C, m->nsp);
 }
 
-/* production rates by the species-wise gather over each species' reactions */
+/* production rates by the species-wise gather over each species' reactions, in blocks of
+   PR_GATHER_BLOCK entries: block sums from 0.0 in list order, added in list order */
 static void gather_wdot(const pr_mech_t* m, const double* q, double* wdot) {
   for (int k = 0; k < m->nsp; ++k) {
+    const int e0 = m->inc_start[k], e1 = m->inc_start[k + 1];
     double w = 0.0;
-    for (int e = m->inc_start[k]; e < m->inc_start[k + 1]; ++e) w += (double)m->inc_nu[e] * q[m->inc_r[e]];
+    for (int b = e0; b < e1; b += PR_GATHER_BLOCK) {
+      const int be = (b + PR_GATHER_BLOCK < e1) ? b + PR_GATHER_BLOCK : e1;
+      double p = 0.0;
+      for (int e = b; e < be; ++e) p += (double)m->inc_nu[e] * q[m->inc_r[e]];
+      w = (b == e0) ? p : w + p;
+    }
     wdot[k] = w;
   }
 }

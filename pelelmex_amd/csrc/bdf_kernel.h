@@ -477,6 +477,10 @@ struct Smem {
   const double* effval;
   const int* incs;
   const int* inc;
+  const int* incq;       // byte offset of each incidence entry's rate slot in q
+  const double* incnu;   // its stoichiometric coefficient
+  const int* kblk;       // first gather block of each species; kblk[N] = number of blocks
+  const int* blk;        // first incidence entry of each gather block (blk[nblocks] = ninc)
   double* Js;    // global scratch slot: (N+1)^2 row-major saved Jacobian
   double* jrec;  // global scratch slot: 4 R per-reaction Jacobian records (kf, kr, dqdM, dqdT)
   double* sC;    // N concentrations
@@ -586,27 +590,32 @@ __device__ __forceinline__ void net_rates(const Smem<N, R>& S, int lane, double 
     }
   }
   group_sync();
+  // Gather in PR_GATHER_BLOCK-entry blocks (mech_common.h): lane l sums blocks l, l + G, ...,
+  // each from 0.0 in list order (up to PR_GATHER_BLOCK entries, the absent ones masked off),
+  // into the species arrays, which are dead once the rates are formed; then each species lane
+  // adds its blocks in order. The longest lists (H, OH, O) no longer set the loop length alone.
+  double* const part = S.sC;
+  const int nblk = S.kblk[N];
+  const unsigned char* const qb = reinterpret_cast<const unsigned char*>(S.q);
+  for (int b = lane; b - lane < nblk; b += G) {
+    if (b < nblk) {
+      const int e0 = S.blk[b], ne = S.blk[b + 1] - e0;
+      double p = 0.0;
+#pragma unroll
+      for (int j = 0; j < PR_GATHER_BLOCK; ++j)
+        if (j < ne) p += S.incnu[e0 + j] * *reinterpret_cast<const double*>(qb + S.incq[e0 + j]);
+      part[b] = p;
+    }
+  }
+  group_sync();
 #pragma unroll
   for (int s = 0; s < RPL; ++s) {
     const int i = lane + G * s;
     w[s] = 0.0;
     if (i < N) {
-      // summed in incidence order; the table reads are issued four at a time
-      double a = 0.0;
-      const int e1 = S.incs[i + 1];
-      int e = S.incs[i];
-      for (; e + 4 <= e1; e += 4) {
-        const int a0 = S.inc[e], a1 = S.inc[e + 1], a2 = S.inc[e + 2], a3 = S.inc[e + 3];
-        const double q0 = S.q[a0 & 0xFFFF], q1 = S.q[a1 & 0xFFFF], q2 = S.q[a2 & 0xFFFF], q3 = S.q[a3 & 0xFFFF];
-        a += (double)(a0 >> 16) * q0;
-        a += (double)(a1 >> 16) * q1;
-        a += (double)(a2 >> 16) * q2;
-        a += (double)(a3 >> 16) * q3;
-      }
-      for (; e < e1; ++e) {
-        const int rn = S.inc[e];
-        a += (double)(rn >> 16) * S.q[rn & 0xFFFF];
-      }
+      const int b0 = S.kblk[i], b1 = S.kblk[i + 1];
+      double a = (b1 > b0) ? part[b0] : 0.0;
+      for (int b = b0 + 1; b < b1; ++b) a += part[b];
       w[s] = a;
     }
   }

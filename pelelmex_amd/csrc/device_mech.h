@@ -37,10 +37,17 @@ struct SpRec {
 // Blob layout (byte offsets, 16-byte aligned sections):
 //   SpRec sp[nsp] | RxRec rx[nr] | int eff_sp[neff] | double eff_val[neff]
 //   | int inc_start[nsp+1] | int inc_rnu[ninc]  (reaction | nu << 16, nu as signed 16-bit)
+//   | int inc_qoff[ninc] (byte offset of the reaction's rate slot) | double inc_nud[ninc] (nu)
+//   | int blk_of_sp[nsp+1] (first gather block of each species; [nsp] = block count)
+//   | int blk_start[max_blocks+1] (first incidence entry of each block)
+// The gather blocks cut every species' incidence list into PR_GATHER_BLOCK-entry pieces, so
+// there are at most ninc / PR_GATHER_BLOCK + nsp of them.
 struct BlobLayout {
   int nsp, nr, neff, ninc;
-  int off_sp, off_rx, off_effsp, off_effval, off_incs, off_inc, bytes;
+  int off_sp, off_rx, off_effsp, off_effval, off_incs, off_inc, off_incq, off_incnu, off_kblk, off_blk, bytes;
 };
+
+__host__ __device__ inline int gather_blocks_max(int nsp, int ninc) { return ninc / PR_GATHER_BLOCK + nsp + 1; }
 
 __host__ __device__ inline BlobLayout blob_layout(int nsp, int nr, int neff, int ninc) {
   BlobLayout L;
@@ -52,6 +59,10 @@ __host__ __device__ inline BlobLayout blob_layout(int nsp, int nr, int neff, int
   L.off_effval = o; o = PR_AL16(o + (neff > 0 ? neff : 1) * 8);
   L.off_incs = o;   o = PR_AL16(o + (nsp + 1) * 4);
   L.off_inc = o;    o = PR_AL16(o + (ninc > 0 ? ninc : 1) * 4);
+  L.off_incq = o;   o = PR_AL16(o + (ninc > 0 ? ninc : 1) * 4);
+  L.off_incnu = o;  o = PR_AL16(o + (ninc > 0 ? ninc : 1) * 8);
+  L.off_kblk = o;   o = PR_AL16(o + (nsp + 1) * 4);
+  L.off_blk = o;    o = PR_AL16(o + (gather_blocks_max(nsp, ninc) + 1) * 4);
   L.bytes = o;
   return L;
 }

@@ -42,6 +42,13 @@ typedef struct {
   const int* inc_nu;       /* [ninc] net stoichiometric coefficient */
 } pr_mech_t;
 
+/* Net production rate of species k: wdot_k = sum over its incidence list (reaction order) of
+ * nu_e q_e. The list is cut into consecutive blocks of PR_GATHER_BLOCK entries; each block is
+ * summed from 0.0 in list order, and the block sums are added in list order to the first one. A
+ * list of at most PR_GATHER_BLOCK entries is thus one sequential sum. The blocks of all species
+ * are independent, which spreads a long list (H, OH, O) over several GPU lanes. */
+#define PR_GATHER_BLOCK 8
+
 /* Physical constants (CGS). */
 #define PR_RU 8.31446261815324e7   /* erg / (mol K) */
 #define PR_PATM 1013250.0          /* dyn / cm^2 */

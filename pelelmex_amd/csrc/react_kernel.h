@@ -55,6 +55,10 @@ __device__ __forceinline__ double* setup_views(Smem<N, R>& S, unsigned char* sme
   S.effval = reinterpret_cast<const double*>(smem + BL.off_effval);
   S.incs = reinterpret_cast<const int*>(smem + BL.off_incs);
   S.inc = reinterpret_cast<const int*>(smem + BL.off_inc);
+  S.incq = reinterpret_cast<const int*>(smem + BL.off_incq);
+  S.incnu = reinterpret_cast<const double*>(smem + BL.off_incnu);
+  S.kblk = reinterpret_cast<const int*>(smem + BL.off_kblk);
+  S.blk = reinterpret_cast<const int*>(smem + BL.off_blk);
   double* scr = reinterpret_cast<double*>(smem + BL.bytes) + (long)grp * group_doubles<N, R, G>();
   S.sC = scr + CTL_DOUBLES;
   S.sG = S.sC + N;

@@ -250,6 +250,23 @@ std::vector<unsigned char> pack_blob(const pr_mech_t* m) {
   auto* inc = reinterpret_cast<int*>(b.data() + L.off_inc);
   for (int k = 0; k <= m->nsp; ++k) incs[k] = m->inc_start[k];
   for (int e = 0; e < m->ninc; ++e) inc[e] = (slot_of[m->inc_r[e]] & 0xFFFF) | (m->inc_nu[e] * 65536);
+  // the production-rate gather: per entry the byte offset of its rate slot and nu as a double,
+  // and the incidence lists cut into PR_GATHER_BLOCK-entry blocks
+  auto* incq = reinterpret_cast<int*>(b.data() + L.off_incq);
+  auto* incnu = reinterpret_cast<double*>(b.data() + L.off_incnu);
+  auto* kblk = reinterpret_cast<int*>(b.data() + L.off_kblk);
+  auto* blk = reinterpret_cast<int*>(b.data() + L.off_blk);
+  for (int e = 0; e < m->ninc; ++e) {
+    incq[e] = slot_of[m->inc_r[e]] * (int)sizeof(double);
+    incnu[e] = (double)m->inc_nu[e];
+  }
+  int nb = 0;
+  for (int k = 0; k < m->nsp; ++k) {
+    kblk[k] = nb;
+    for (int e = m->inc_start[k]; e < m->inc_start[k + 1]; e += PR_GATHER_BLOCK) blk[nb++] = e;
+  }
+  kblk[m->nsp] = nb;
+  blk[nb] = m->ninc;
   return b;
 }
 
@@ -485,6 +502,15 @@ int pr_create(const char* integrator, const char* mechanism, int reactor_type, i
   }
   std::vector<unsigned char> blob = pack_blob(m);
   h->blob_bytes = (int)blob.size();
+  {
+    // the gather's block sums are held in the group's 5 N species doubles of LDS
+    const prk::BlobLayout L = prk::blob_layout(m->nsp, m->nr, m->neff, m->ninc);
+    const int nb = reinterpret_cast<const int*>(blob.data() + L.off_kblk)[m->nsp];
+    if (nb > 5 * m->nsp) {
+      delete h;
+      return fail(PR_ERR_UNSUPPORTED, "mechanism has more production-rate gather blocks than 5 per species");
+    }
+  }
   // per-component absolute tolerances: ode.atol everywhere until set_typ_vals_ode scales them
   // (the kernels always read this array, so a captured graph sees later typical values)
   const std::vector<double> atol0((size_t)m->nsp + 1, K.atol);

@@ -19,6 +19,7 @@ from __future__ import annotations
 from .mechanisms import load
 
 NASA_HCP = 18          # cp/R (4 fma) + h/RT (5 fma)
+GATHER_BLOCK = 8       # PR_GATHER_BLOCK of csrc/mech_common.h: block sums of the production rates
 NASA_FULL = 35         # + s/R (4 fma, mul, add) + g (sub) + dcp (3 fma)
 HY2T_ITERS = 2
 
@@ -52,18 +53,22 @@ def static_counts(mech_name):
     n = N + 1
     ninc = 0
     row_work = 0
+    combine = 0                             # additions of a species' block sums
     for k, s in enumerate(m.species):
+        nk = 0
         for r in m.reactions:
             nu = r.prod.count(s) - r.reac.count(s)
             if nu != 0:
                 ninc += 1
+                nk += 1
                 row_work += 8 + 3 * (len(r.reac) + len(r.prod))
                 if r.third is not None:
                     row_work += 3 + 2 * len([e for e in r.eff.values() if e != 1.0])
+        combine += max(-(-nk // GATHER_BLOCK) - 1, 0)
     rx_rhs = sum(_reaction_flops(r, False) for r in m.reactions)
     rx_jac = sum(_reaction_flops(r, True) for r in m.reactions)
     F_rhs = (N + 3) + HY2T_ITERS * ((NASA_HCP + 8) * N + 5) + 5 + (NASA_FULL + 5) * N + rx_rhs \
-        + 2 * ninc + 2 * N + 4 * N + 2
+        + 2 * ninc + combine + 2 * N + 4 * N + 2
     F_jac = 5 + (NASA_FULL + 6) * N + N + rx_jac + row_work + (3 * N + 1) * N + 2 * N + 12 * N + 4 \
         + (2 * N + 4) * n
     lu = 2 * n * n
