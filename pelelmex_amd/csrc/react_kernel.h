@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
   Smem<N, R> S;
   int lane = threadIdx.x % G;  // re-derived opaquely each iteration (see the loop head)
   const int grp = threadIdx.x / G;
-  Ctl& c = *reinterpret_cast<Ctl*>(setup_views<N, R, G>(S, smem, BL, grp, P.gscr));
+  Ctl* ct = reinterpret_cast<Ctl*>(setup_views<N, R, G>(S, smem, BL, grp, P.gscr));
 
   long nwork = P.ncells;
   if (P.ncells_dev) {
@@ -111,8 +111,8 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
   }
   const int qmax = P.max_order < 1 ? 1 : (P.max_order > Q ? Q : P.max_order);
 
-  c.s_cells = c.s_nst = c.s_nfe = c.s_nje = c.s_nset = c.s_nni = c.s_netf = c.s_ncfn = c.s_fail = 0u;
-  c.phase = PH_FETCH;
+  ct->s_cells = ct->s_nst = ct->s_nfe = ct->s_nje = ct->s_nset = ct->s_nni = ct->s_netf = ct->s_ncfn = ct->s_fail = 0u;
+  ct->phase = PH_FETCH;
 
   // vectors kept in registers (rows lane, lane + G, ...)
   double zn[Q + 1][RPL], ewt[RPL], acor[RPL], yv[RPL], f[RPL], src[RPL], atol_r[RPL], neg[RPL];
@@ -135,9 +135,9 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
   };
 
   auto finish = [&](int code, const double (&out)[RPL]) __attribute__((always_inline)) {
-    const BoxDesc& B = box_of(c.box);
+    const BoxDesc& B = box_of(ct->box);
     int i, j, k;
-    cell_ijk(B, c.cell, i, j, k);
+    cell_ijk(B, ct->cell, i, j, k);
     const long offY = aoff(B.rY, i, j, k), offT = aoff(B.T, i, j, k);
 #pragma unroll
     for (int s = 0; s < RPL; ++s) {
@@ -146,15 +146,15 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
       else if (ii == N) B.T.p[offT] = out[s];
     }
     if (lane == 0) {
-      B.rH.p[aoff(B.rH, i, j, k)] = c.rhoh0 + c.tout * c.rhohsrc;
-      B.FC.p[aoff(B.FC, i, j, k)] = (double)c.nfe;
-      if (B.status) B.status[c.cell] = code;
+      B.rH.p[aoff(B.rH, i, j, k)] = ct->rhoh0 + ct->tout * ct->rhohsrc;
+      B.FC.p[aoff(B.FC, i, j, k)] = (double)ct->nfe;
+      if (B.status) B.status[ct->cell] = code;
     }
-    c.s_cells += 1u;
-    c.s_nst += c.nst; c.s_nfe += c.nfe; c.s_nje += c.nje; c.s_nset += c.nset; c.s_nni += c.nni;
-    c.s_netf += c.netf; c.s_ncfn += c.ncfn;
-    if (code != ST_OK) c.s_fail += 1u;
-    c.phase = PH_FETCH;
+    ct->s_cells += 1u;
+    ct->s_nst += ct->nst; ct->s_nfe += ct->nfe; ct->s_nje += ct->nje; ct->s_nset += ct->nset; ct->s_nni += ct->nni;
+    ct->s_netf += ct->netf; ct->s_ncfn += ct->ncfn;
+    if (code != ST_OK) ct->s_fail += 1u;
+    ct->phase = PH_FETCH;
   };
 
   auto set_ewt = [&](const double (&y)[RPL]) __attribute__((always_inline)) {
@@ -165,15 +165,15 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
   // reciprocals of the order the step control divides by, formed once per order change (the same
   // correctly rounded values the divisions at each use gave)
   auto set_recips = [&]() __attribute__((always_inline)) {
-    const double q = (double)c.q;
-    c.rq = 1.0 / q;
-    c.rq1 = 1.0 / (q + 1.0);
-    c.rq2 = 1.0 / (q + 2.0);
+    const double q = (double)ct->q;
+    ct->rq = 1.0 / q;
+    ct->rq1 = 1.0 / (q + 1.0);
+    ct->rq2 = 1.0 / (q + 2.0);
   };
 
   auto rescale = [&]() __attribute__((always_inline)) {
-    const double eta = c.eta;
-    const int q = c.q;
+    const double eta = ct->eta;
+    const int q = ct->q;
     double factor = eta;
 #pragma unroll
     for (int j = 1; j <= Q; ++j) {
@@ -183,14 +183,14 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
         factor *= eta;
       }
     }
-    c.h = c.hscale * eta;
-    c.hscale = c.h;
-    c.nscon = 0;
+    ct->h = ct->hscale * eta;
+    ct->hscale = ct->h;
+    ct->nscon = 0;
   };
 
   auto predict = [&]() __attribute__((always_inline)) {
-    c.tn += c.h;
-    const int q = c.q;
+    ct->tn += ct->h;
+    const int q = ct->q;
 #pragma unroll
     for (int k = 1; k <= Q; ++k)
 #pragma unroll
@@ -201,8 +201,8 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
   };
 
   auto restore = [&]() __attribute__((always_inline)) {
-    c.tn = c.saved_t;
-    const int q = c.q;
+    ct->tn = ct->saved_t;
+    const int q = ct->q;
 #pragma unroll
     for (int k = 1; k <= Q; ++k)
 #pragma unroll
@@ -216,12 +216,12 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
   // Computed in registers (loops unrolled to the maximum order, predicated on q) and written
   // back once; the operations and their order are those of the loop form.
   auto set_bdf = [&]() __attribute__((always_inline)) {
-    const int q = c.q;
-    const double h = c.h;
+    const int q = ct->q;
+    const double h = ct->h;
     double tau[Q + 1], l[Q + 1];
 #pragma unroll
     for (int i = 0; i <= Q; ++i) {
-      tau[i] = c.tau[i];
+      tau[i] = ct->tau[i];
       l[i] = 0.0;
     }
     double alpha0 = -1.0, alpha0_hat = -1.0, xi_inv = 1.0, xistar_inv = 1.0, hsum = h;
@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
           for (int i = j; i >= 1; --i) l[i] += l[i - 1] * xi_inv;
         }
       }
-      alpha0 -= c.rq;
+      alpha0 -= ct->rq;
       xistar_inv = -l[1] - alpha0;
       double tq1 = 0.0;
 #pragma unroll
@@ -257,17 +257,17 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
       if (i == q) lq = l[i];
     const double A1 = 1.0 - alpha0_hat + alpha0;
     const double A2 = 1.0 + q * A1;
-    c.tq[2] = fabs(A1 / (alpha0 * A2));
-    c.tq[5] = fabs(A2 * xistar_inv / (lq * xi_inv));
-    if (c.qwait == 1) {
+    ct->tq[2] = fabs(A1 / (alpha0 * A2));
+    ct->tq[5] = fabs(A2 * xistar_inv / (lq * xi_inv));
+    if (ct->qwait == 1) {
       if (q > 1) {
         double C = xistar_inv / lq;
-        double A3 = alpha0 + c.rq;
+        double A3 = alpha0 + ct->rq;
         double A4 = alpha0_hat + xi_inv;
         double Cpinv = (1.0 - A4 + A3) / A3;
-        c.tq[1] = fabs(C * Cpinv);
+        ct->tq[1] = fabs(C * Cpinv);
       } else {
-        c.tq[1] = 1.0;
+        ct->tq[1] = 1.0;
       }
       double tqq = 0.0;
 #pragma unroll
@@ -275,29 +275,29 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
         if (j == q) tqq = tau[j];
       hsum += tqq;
       xi_inv = h / hsum;
-      double A5 = alpha0 - c.rq1;
+      double A5 = alpha0 - ct->rq1;
       double A6 = alpha0_hat - xi_inv;
       double Cppinv = (1.0 - A6 + A5) / A2;
-      c.tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
+      ct->tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
     }
-    c.tq[4] = 0.1 / c.tq[2];
+    ct->tq[4] = 0.1 / ct->tq[2];
 #pragma unroll
-    for (int i = 0; i <= Q; ++i) c.l[i] = l[i];
-    c.rl1 = 1.0 / l[1];
-    c.gamma = h * c.rl1;
-    if (c.nst == 0) c.gammap = c.gamma;
-    c.gamrat = (c.nst > 0) ? c.gamma / c.gammap : 1.0;
+    for (int i = 0; i <= Q; ++i) ct->l[i] = l[i];
+    ct->rl1 = 1.0 / l[1];
+    ct->gamma = h * ct->rl1;
+    if (ct->nst == 0) ct->gammap = ct->gamma;
+    ct->gamrat = (ct->nst > 0) ? ct->gamma / ct->gammap : 1.0;
   };
 
   auto decrease_bdf = [&]() __attribute__((always_inline)) {
-    const int q = c.q;
-    double* l = c.l;
+    const int q = ct->q;
+    double* l = ct->l;
     for (int i = 0; i <= Q; ++i) l[i] = 0.0;
     l[2] = 1.0;
     double hsum = 0.0;
     for (int j = 1; j <= q - 2; ++j) {
-      hsum += c.tau[j];
-      double xi = hsum / c.hscale;
+      hsum += ct->tau[j];
+      double xi = hsum / ct->hscale;
       for (int i = j + 2; i >= 2; --i) l[i] = l[i] * xi + l[i - 1];
     }
     double zq[RPL];
@@ -318,15 +318,15 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
   };
 
   auto increase_bdf = [&]() __attribute__((always_inline)) {
-    const int q = c.q, L = c.L;
-    double* l = c.l;
+    const int q = ct->q, L = ct->L;
+    double* l = ct->l;
     for (int i = 0; i <= Q; ++i) l[i] = 0.0;
-    double alpha1 = 1.0, prod = 1.0, xiold = 1.0, alpha0 = -1.0, hsum = c.hscale;
+    double alpha1 = 1.0, prod = 1.0, xiold = 1.0, alpha0 = -1.0, hsum = ct->hscale;
     l[2] = 1.0;
     if (q > 1) {
       for (int j = 1; j < q; ++j) {
-        hsum += c.tau[j + 1];
-        double xi = hsum / c.hscale;
+        hsum += ct->tau[j + 1];
+        double xi = hsum / ct->hscale;
         prod *= xi;
         alpha0 -= 1.0 / (j + 1);
         alpha1 += 1.0 / xi;
@@ -360,15 +360,15 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
   };
 
   auto adjust_params = [&]() __attribute__((always_inline)) {
-    if (c.qprime != c.q) {
-      const int dq = c.qprime - c.q;
-      if (!(c.q == 2 && dq != 1)) {
+    if (ct->qprime != ct->q) {
+      const int dq = ct->qprime - ct->q;
+      if (!(ct->q == 2 && dq != 1)) {
         if (dq == 1) increase_bdf();
         else if (dq == -1) decrease_bdf();
       }
-      c.q = c.qprime;
-      c.L = c.q + 1;
-      c.qwait = c.L;
+      ct->q = ct->qprime;
+      ct->L = ct->q + 1;
+      ct->qwait = ct->L;
       set_recips();
     }
     rescale();
@@ -377,42 +377,42 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
   auto attempt = [&]() __attribute__((always_inline)) {
     predict();
     set_bdf();
-    const int nflag = c.nflag;
-    c.convfail = (nflag == NF_FIRST || nflag == NF_ERR) ? CF_NONE : CF_OTHER;
-    c.callSetup = (nflag == NF_CONV) || (nflag == NF_ERR) || (c.nst == 0) || (c.nst >= c.nstlp + 20) ||
-                  (fabs(c.gamrat - 1.0) > 0.3);
-    c.jbad = 0;
-    c.mnewt = 0;
+    const int nflag = ct->nflag;
+    ct->convfail = (nflag == NF_FIRST || nflag == NF_ERR) ? CF_NONE : CF_OTHER;
+    ct->callSetup = (nflag == NF_CONV) || (nflag == NF_ERR) || (ct->nst == 0) || (ct->nst >= ct->nstlp + 20) ||
+                  (fabs(ct->gamrat - 1.0) > 0.3);
+    ct->jbad = 0;
+    ct->mnewt = 0;
 #pragma unroll
     for (int s = 0; s < RPL; ++s) {
       acor[s] = 0.0;
       yv[s] = zn[0][s];
     }
-    c.t_eval = c.tn;
-    c.phase = PH_NEWTON;
+    ct->t_eval = ct->tn;
+    ct->phase = PH_NEWTON;
   };
 
   auto begin_step = [&]() __attribute__((always_inline)) {
-    if (c.nst > 0) set_ewt(zn[0]);
-    if (c.nstloc >= P.max_steps) {
+    if (ct->nst > 0) set_ewt(zn[0]);
+    if (ct->nstloc >= P.max_steps) {
       finish(ST_TOO_MUCH_WORK, zn[0]);
       return;
     }
-    if (c.nst > 0 && c.hprime != c.h) adjust_params();
-    c.saved_t = c.tn;
-    c.ncf = 0;
-    c.nef = 0;
-    c.nflag = NF_FIRST;
+    if (ct->nst > 0 && ct->hprime != ct->h) adjust_params();
+    ct->saved_t = ct->tn;
+    ct->ncf = 0;
+    ct->nef = 0;
+    ct->nflag = NF_FIRST;
     attempt();
   };
 
   auto start_integration = [&]() __attribute__((always_inline)) {
-    const double h = c.h;
+    const double h = ct->h;
 #pragma unroll
     for (int s = 0; s < RPL; ++s) zn[1][s] *= h;
-    c.hscale = h;
-    c.hprime = h;
-    c.nstloc = 0;
+    ct->hscale = h;
+    ct->hprime = h;
+    ct->nstloc = 0;
     begin_step();
   };
 
@@ -423,7 +423,12 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
     // lane-derived masks and addresses are formed inside each iteration: hoisted out of the loop
     // they would be held (and spilled) for the whole kernel
     lane = opaque(lane);
-    if (uni<G>(c.phase) == PH_FETCH) {
+    // At 64 lanes the group's LDS views and slot addresses are re-derived the same way, from an
+    // opaque group index: as loop invariants the many addresses into the control block and the
+    // species arrays were held (and spilled) for the whole kernel (-1.3 % on n-dodecane; at 32
+    // lanes it costs more registers than it saves, +2 % on drm19)
+    if constexpr (G == 64) ct = reinterpret_cast<Ctl*>(setup_views<N, R, G>(S, smem, BL, opaque(grp), P.gscr));
+    if (uni<G>(ct->phase) == PH_FETCH) {
       unsigned int cl = 0;
       if (lane == 0) cl = atomicAdd(P.queue, 1u);
       cl = __shfl(cl, 0, G);
@@ -443,9 +448,9 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
       const BoxDesc& B = box_of(b);
       int i, j, k;
       cell_ijk(B, (int)cl, i, j, k);
-      c.cell = (int)cl;
-      c.box = b;
-      c.tout = B.dtv ? B.dtv[cl] : B.dt;
+      ct->cell = (int)cl;
+      ct->box = b;
+      ct->tout = B.dtv ? B.dtv[cl] : B.dt;
       if (B.has_mask) {
         long mo = (long)(i - B.mask.bx) + (long)(j - B.mask.by) * B.mask.js + (long)(k - B.mask.bz) * B.mask.ks;
         if (B.mask.p[mo] == -1) {
@@ -478,30 +483,30 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
 #pragma unroll
         for (int j2 = 1; j2 <= Q; ++j2) zn[j2][s] = 0.0;
       }
-      c.rhoh0 = B.rH.p[aoff(B.rH, i, j, k)];
-      c.rhohsrc = B.rHsrc.p[aoff(B.rHsrc, i, j, k)];
-      c.nfe = c.nje = c.nset = c.nni = c.netf = c.ncfn = 0;
-      c.nst = c.nstlp = c.nstlj = 0;
-      c.q = 1; c.L = 2; c.qwait = 2; c.qprime = 1;
-      c.rq = 1.0; c.rq1 = 0.5; c.rq2 = 1.0 / 3.0;
-      c.etamax = 10000.0; c.nscon = 0; c.saved_tq5 = 0.0; c.crate = 1.0; c.delp = 0.0;
-      c.gamrat = 1.0; c.gammap = 1.0; c.tn = 0.0; c.jcur = 0; c.eta = 1.0;
-      for (int j2 = 0; j2 <= Q; ++j2) c.tau[j2] = 0.0;
-      for (int j2 = 0; j2 < 6; ++j2) c.tq[j2] = 0.0;
+      ct->rhoh0 = B.rH.p[aoff(B.rH, i, j, k)];
+      ct->rhohsrc = B.rHsrc.p[aoff(B.rHsrc, i, j, k)];
+      ct->nfe = ct->nje = ct->nset = ct->nni = ct->netf = ct->ncfn = 0;
+      ct->nst = ct->nstlp = ct->nstlj = 0;
+      ct->q = 1; ct->L = 2; ct->qwait = 2; ct->qprime = 1;
+      ct->rq = 1.0; ct->rq1 = 0.5; ct->rq2 = 1.0 / 3.0;
+      ct->etamax = 10000.0; ct->nscon = 0; ct->saved_tq5 = 0.0; ct->crate = 1.0; ct->delp = 0.0;
+      ct->gamrat = 1.0; ct->gammap = 1.0; ct->tn = 0.0; ct->jcur = 0; ct->eta = 1.0;
+      for (int j2 = 0; j2 <= Q; ++j2) ct->tau[j2] = 0.0;
+      for (int j2 = 0; j2 < 6; ++j2) ct->tq[j2] = 0.0;
       set_ewt(zn[0]);
-      c.t_eval = 0.0;
-      c.guard = 0;
-      c.phase = PH_F0;
+      ct->t_eval = 0.0;
+      ct->guard = 0;
+      ct->phase = PH_F0;
     }
-    if (++c.guard > guard_max) {
+    if (++ct->guard > guard_max) {
       finish(ST_GUARD, zn[0]);
       continue;
     }
 
-    rhs_eval<N, R, G>(S, lane, c.t_eval, yv, src, c.rhoh0, c.rhohsrc, f);
+    rhs_eval<N, R, G>(S, lane, ct->t_eval, yv, src, ct->rhoh0, ct->rhohsrc, f);
     clobber();
-    c.nfe += 1;
-    const int phase = uni<G>(c.phase);
+    ct->nfe += 1;
+    const int phase = uni<G>(ct->phase);
 
     if (phase == PH_F0) {
       double hinv = 0.0;
@@ -515,98 +520,98 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
         }
       }
       hinv = gmax<G>(hinv);
-      const double tout = c.tout;
+      const double tout = ct->tout;
       const double tround = DBL_EPSILON * fabs(tout);
       const double hlb = 100.0 * tround;
       double hub = 0.1 * fabs(tout);
       if (hub * hinv > 1.0) hub = 1.0 / hinv;
       const double hg = sqrt(hlb * hub);
-      c.hlb = hlb;
-      c.hub = hub;
-      c.hg = hg;
+      ct->hlb = hlb;
+      ct->hub = hub;
+      ct->hg = hg;
       if (hub < hlb) {
-        c.h = hg;
+        ct->h = hg;
         start_integration();
       } else {
-        c.count1 = 1;
+        ct->count1 = 1;
 #pragma unroll
         for (int s = 0; s < RPL; ++s) yv[s] = zn[0][s] + hg * zn[1][s];
-        c.t_eval = hg;
-        c.phase = PH_HIN;
+        ct->t_eval = hg;
+        ct->phase = PH_HIN;
       }
     } else if (phase == PH_HIN) {
-      const double hg = c.hg, hub = c.hub;
+      const double hg = ct->hg, hub = ct->hub;
       double tmp[RPL];
 #pragma unroll
       for (int s = 0; s < RPL; ++s) tmp[s] = (f[s] - zn[1][s]) / hg;
       const double yddnrm = wrms<N, R, G>(S, lane, tmp, ewt);
       double hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
       bool done = false;
-      if (c.count1 == 4) {
+      if (ct->count1 == 4) {
         done = true;
       } else {
         double hrat = hnew / hg;
         if (hrat > 0.5 && hrat < 2.0) done = true;
-        else if (c.count1 > 1 && hrat > 2.0) {
+        else if (ct->count1 > 1 && hrat > 2.0) {
           hnew = hg;
           done = true;
         }
       }
       if (done) {
         double h0 = 0.5 * hnew;
-        if (h0 < c.hlb) h0 = c.hlb;
+        if (h0 < ct->hlb) h0 = ct->hlb;
         if (h0 > hub) h0 = hub;
-        c.h = h0;
+        ct->h = h0;
         start_integration();
       } else {
-        c.hg = hnew;
-        c.count1 += 1;
+        ct->hg = hnew;
+        ct->count1 += 1;
 #pragma unroll
         for (int s = 0; s < RPL; ++s) yv[s] = zn[0][s] + hnew * zn[1][s];
-        c.t_eval = hnew;
+        ct->t_eval = hnew;
       }
     } else if (phase == PH_RESTART) {
-      const double h = c.h;
+      const double h = ct->h;
 #pragma unroll
       for (int s = 0; s < RPL; ++s) zn[1][s] = h * f[s];
       attempt();
     } else {  // PH_NEWTON: f = f(tn, zn0 + acor)
       double res[RPL];
       {
-        const double rl1 = c.rl1, gamma = c.gamma;
+        const double rl1 = ct->rl1, gamma = ct->gamma;
 #pragma unroll
         for (int s = 0; s < RPL; ++s) res[s] = rl1 * zn[1][s] + acor[s] - gamma * f[s];
       }
       bool singular = false;
-      if (uni<G>(c.callSetup)) {
-        const int cf = c.jbad ? CF_BADJ : c.convfail;
-        const double dgamma = fabs(c.gamma / c.gammap - 1.0);
-        const bool jnew = (c.nst == 0) || (c.nst >= c.nstlj + 51) || (cf == CF_BADJ && dgamma < 0.2) || (cf == CF_OTHER);
+      if (uni<G>(ct->callSetup)) {
+        const int cf = ct->jbad ? CF_BADJ : ct->convfail;
+        const double dgamma = fabs(ct->gamma / ct->gammap - 1.0);
+        const bool jnew = (ct->nst == 0) || (ct->nst >= ct->nstlj + 51) || (cf == CF_BADJ && dgamma < 0.2) || (cf == CF_OTHER);
         if (jnew) {
-          c.nje += 1;
-          c.nstlj = c.nst;
-          c.jcur = 1;
-          jac_eval<N, R, G>(S, lane, yv, src, c.rhohsrc);
+          ct->nje += 1;
+          ct->nstlj = ct->nst;
+          ct->jcur = 1;
+          jac_eval<N, R, G>(S, lane, yv, src, ct->rhohsrc);
           clobber();
         } else {
-          c.jcur = 0;
+          ct->jcur = 0;
         }
         {
           double lu[RPL][n];
-          singular = lu_factor<N, R, G>(S, lane, c.gamma, lu, pst, rinv, permv);
+          singular = lu_factor<N, R, G>(S, lane, ct->gamma, lu, pst, rinv, permv);
           store_lu<N, R, G>(S, lane, lu, pst);
         }
         clobber();
-        c.nset += 1;
-        c.gamrat = 1.0;
-        c.gammap = c.gamma;
-        c.crate = 1.0;
-        c.nstlp = c.nst;
-        c.callSetup = 0;
+        ct->nset += 1;
+        ct->gamrat = 1.0;
+        ct->gammap = ct->gamma;
+        ct->crate = 1.0;
+        ct->nstlp = ct->nst;
+        ct->callSetup = 0;
       }
       int outcome = 0;  // 0 continue, 1 converged, 2 failed
       if (!singular) {
-        c.nni += 1;
+        ct->nni += 1;
         double b[RPL], dlt[RPL];
 #pragma unroll
         for (int s = 0; s < RPL; ++s) b[s] = -res[s];
@@ -616,7 +621,7 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
           lu_solve<N, R, G>(S, lane, lu, pst, rinv, permv, b, dlt);
         }
         clobber();
-        const double gamrat = c.gamrat;
+        const double gamrat = ct->gamrat;
         if (gamrat != 1.0) {
           const double sc = 2.0 / (1.0 + gamrat);
 #pragma unroll
@@ -625,18 +630,18 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
 #pragma unroll
         for (int s = 0; s < RPL; ++s) acor[s] += dlt[s];
         const double del = wrms<N, R, G>(S, lane, dlt, ewt);
-        const int mnewt = c.mnewt;
-        if (mnewt > 0) c.crate = fmax(0.3 * c.crate, del / c.delp);
-        const double dcon = del * fmin(1.0, c.crate) / c.tq[4];
+        const int mnewt = ct->mnewt;
+        if (mnewt > 0) ct->crate = fmax(0.3 * ct->crate, del / ct->delp);
+        const double dcon = del * fmin(1.0, ct->crate) / ct->tq[4];
         if (dcon <= 1.0) {
-          c.acnrm = (mnewt == 0) ? del : wrms<N, R, G>(S, lane, acor, ewt);
-          c.jcur = 0;
+          ct->acnrm = (mnewt == 0) ? del : wrms<N, R, G>(S, lane, acor, ewt);
+          ct->jcur = 0;
           outcome = 1;
-        } else if (mnewt >= 1 && del > 2.0 * c.delp) {
+        } else if (mnewt >= 1 && del > 2.0 * ct->delp) {
           outcome = 2;
         } else {
-          c.delp = del;
-          c.mnewt = mnewt + 1;
+          ct->delp = del;
+          ct->mnewt = mnewt + 1;
           if (mnewt + 1 >= 3) outcome = 2;
         }
       } else {
@@ -646,69 +651,69 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
 #pragma unroll
         for (int s = 0; s < RPL; ++s) yv[s] = zn[0][s] + acor[s];
       } else if (outcome == 2) {
-        if (!c.jcur) {
-          c.callSetup = 1;
-          c.jbad = 1;
-          c.mnewt = 0;
+        if (!ct->jcur) {
+          ct->callSetup = 1;
+          ct->jbad = 1;
+          ct->mnewt = 0;
 #pragma unroll
           for (int s = 0; s < RPL; ++s) {
             acor[s] = 0.0;
             yv[s] = zn[0][s];
           }
         } else {
-          c.ncfn += 1;
+          ct->ncfn += 1;
           restore();
-          c.ncf += 1;
-          c.etamax = 1.0;
-          if (c.ncf == 10) {
+          ct->ncf += 1;
+          ct->etamax = 1.0;
+          if (ct->ncf == 10) {
             finish(ST_CONV_FAILURE, zn[0]);
           } else {
-            c.eta = 0.25;
-            c.nflag = NF_CONV;
+            ct->eta = 0.25;
+            ct->nflag = NF_CONV;
             rescale();
             attempt();
           }
         }
       } else {
-        const double dsm = c.acnrm * c.tq[2];
+        const double dsm = ct->acnrm * ct->tq[2];
         if (dsm <= 1.0) {
           // complete the step
-          c.nst += 1;
-          c.nscon += 1;
-          const int q = c.q;
-          for (int i = q; i >= 2; --i) c.tau[i] = c.tau[i - 1];
-          if (q == 1 && c.nst > 1) c.tau[2] = c.tau[1];
-          c.tau[1] = c.h;
+          ct->nst += 1;
+          ct->nscon += 1;
+          const int q = ct->q;
+          for (int i = q; i >= 2; --i) ct->tau[i] = ct->tau[i - 1];
+          if (q == 1 && ct->nst > 1) ct->tau[2] = ct->tau[1];
+          ct->tau[1] = ct->h;
 #pragma unroll
           for (int j = 0; j <= Q; ++j)
             if (j <= q) {
-              const double lj = c.l[j];
+              const double lj = ct->l[j];
 #pragma unroll
               for (int s = 0; s < RPL; ++s) zn[j][s] += lj * acor[s];
             }
-          c.qwait -= 1;
-          if (c.qwait == 1 && q != qmax) {
+          ct->qwait -= 1;
+          if (ct->qwait == 1 && q != qmax) {
 #pragma unroll
             for (int j = 0; j <= Q; ++j)
               if (j == qmax)
 #pragma unroll
                 for (int s = 0; s < RPL; ++s) zn[j][s] = acor[s];
-            c.saved_tq5 = c.tq[5];
+            ct->saved_tq5 = ct->tq[5];
           }
           // choose the next step size and order
-          if (c.etamax == 1.0) {
-            if (c.qwait < 2) c.qwait = 2;
-            c.qprime = q;
-            c.hprime = c.h;
-            c.eta = 1.0;
+          if (ct->etamax == 1.0) {
+            if (ct->qwait < 2) ct->qwait = 2;
+            ct->qprime = q;
+            ct->hprime = ct->h;
+            ct->eta = 1.0;
           } else {
-            const int L = c.L;
-            const double etaq = 1.0 / (pr_pow(6.0 * dsm, c.rq1) + 1e-6);
-            if (c.qwait != 0) {
-              c.eta = etaq;
-              c.qprime = q;
+            const int L = ct->L;
+            const double etaq = 1.0 / (pr_pow(6.0 * dsm, ct->rq1) + 1e-6);
+            if (ct->qwait != 0) {
+              ct->eta = etaq;
+              ct->qprime = q;
             } else {
-              c.qwait = 2;
+              ct->qwait = 2;
               double etaqm1 = 0.0, etaqp1 = 0.0;
               if (q > 1) {
                 double zq[RPL];
@@ -719,11 +724,11 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
                   if (j == q)
 #pragma unroll
                     for (int s = 0; s < RPL; ++s) zq[s] = zn[j][s];
-                const double ddn = wrms<N, R, G>(S, lane, zq, ewt) * c.tq[1];
-                etaqm1 = 1.0 / (pr_pow(6.0 * ddn, c.rq) + 1e-6);
+                const double ddn = wrms<N, R, G>(S, lane, zq, ewt) * ct->tq[1];
+                etaqm1 = 1.0 / (pr_pow(6.0 * ddn, ct->rq) + 1e-6);
               }
-              if (q != qmax && c.saved_tq5 != 0.0) {
-                const double cquot = (c.tq[5] / c.saved_tq5) * pr_pow(c.h / c.tau[2], (double)L);
+              if (q != qmax && ct->saved_tq5 != 0.0) {
+                const double cquot = (ct->tq[5] / ct->saved_tq5) * pr_pow(ct->h / ct->tau[2], (double)L);
                 double tmp[RPL];
 #pragma unroll
                 for (int s = 0; s < RPL; ++s) tmp[s] = acor[s];
@@ -732,22 +737,22 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
                   if (j == qmax)
 #pragma unroll
                     for (int s = 0; s < RPL; ++s) tmp[s] = acor[s] - cquot * zn[j][s];
-                const double dup = wrms<N, R, G>(S, lane, tmp, ewt) * c.tq[3];
-                etaqp1 = 1.0 / (pr_pow(10.0 * dup, c.rq2) + 1e-6);
+                const double dup = wrms<N, R, G>(S, lane, tmp, ewt) * ct->tq[3];
+                etaqp1 = 1.0 / (pr_pow(10.0 * dup, ct->rq2) + 1e-6);
               }
               const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
               if (etam < 1.5) {
-                c.eta = 1.0;
-                c.qprime = q;
+                ct->eta = 1.0;
+                ct->qprime = q;
               } else if (etam == etaqm1) {
-                c.eta = etaqm1;
-                c.qprime = q - 1;
+                ct->eta = etaqm1;
+                ct->qprime = q - 1;
               } else if (etam == etaq) {
-                c.eta = etaq;
-                c.qprime = q;
+                ct->eta = etaq;
+                ct->qprime = q;
               } else {
-                c.eta = etaqp1;
-                c.qprime = q + 1;
+                ct->eta = etaqp1;
+                ct->qprime = q + 1;
 #pragma unroll
                 for (int j = 0; j <= Q; ++j)
                   if (j == qmax)
@@ -755,18 +760,18 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
                     for (int s = 0; s < RPL; ++s) zn[j][s] = acor[s];
               }
             }
-            if (c.eta < 1.5) {
-              c.eta = 1.0;
-              c.hprime = c.h;
+            if (ct->eta < 1.5) {
+              ct->eta = 1.0;
+              ct->hprime = ct->h;
             } else {
-              c.eta = fmin(c.eta, c.etamax);
-              c.hprime = c.h * c.eta;
-              if (c.qprime < q) c.nscon = 0;
+              ct->eta = fmin(ct->eta, ct->etamax);
+              ct->hprime = ct->h * ct->eta;
+              if (ct->qprime < q) ct->nscon = 0;
             }
           }
-          c.etamax = 10.0;
-          c.nstloc += 1;
-          const double tn = c.tn, h = c.h, tout = c.tout;
+          ct->etamax = 10.0;
+          ct->nstloc += 1;
+          const double tn = ct->tn, h = ct->h, tout = ct->tout;
           if ((tn - tout) * h >= 0.0) {
             const double sfac = (tout - tn) / h;
             double out[RPL];
@@ -788,40 +793,40 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
           }
         } else {
           // error test failure
-          c.nef += 1;
-          c.netf += 1;
-          c.nflag = NF_ERR;
+          ct->nef += 1;
+          ct->netf += 1;
+          ct->nflag = NF_ERR;
           restore();
-          if (c.nef == 7) {
+          if (ct->nef == 7) {
             finish(ST_ERR_FAILURE, zn[0]);
           } else {
-            c.etamax = 1.0;
-            if (c.nef <= 3) {
-              double eta = 1.0 / (pr_pow(6.0 * dsm, c.rq1) + 1e-6);
+            ct->etamax = 1.0;
+            if (ct->nef <= 3) {
+              double eta = 1.0 / (pr_pow(6.0 * dsm, ct->rq1) + 1e-6);
               eta = fmax(0.1, eta);
-              if (c.nef >= 2) eta = fmin(eta, 0.2);
-              c.eta = eta;
+              if (ct->nef >= 2) eta = fmin(eta, 0.2);
+              ct->eta = eta;
               rescale();
               attempt();
-            } else if (c.q > 1) {
-              c.eta = 0.1;
+            } else if (ct->q > 1) {
+              ct->eta = 0.1;
               decrease_bdf();
-              c.L = c.q;
-              c.q -= 1;
-              c.qwait = c.L;
+              ct->L = ct->q;
+              ct->q -= 1;
+              ct->qwait = ct->L;
               set_recips();
               rescale();
               attempt();
             } else {
-              c.eta = 0.1;
-              c.h *= 0.1;
-              c.hscale = c.h;
-              c.qwait = 10;
-              c.nscon = 0;
+              ct->eta = 0.1;
+              ct->h *= 0.1;
+              ct->hscale = ct->h;
+              ct->qwait = 10;
+              ct->nscon = 0;
 #pragma unroll
               for (int s = 0; s < RPL; ++s) yv[s] = zn[0][s];
-              c.t_eval = c.tn;
-              c.phase = PH_RESTART;
+              ct->t_eval = ct->tn;
+              ct->phase = PH_RESTART;
             }
           }
         }
@@ -830,15 +835,15 @@ __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(W,
   }
 
   if (lane == 0) {
-    atomicAdd(&P.stats[0], (unsigned long long)c.s_cells);
-    atomicAdd(&P.stats[1], (unsigned long long)c.s_nst);
-    atomicAdd(&P.stats[2], (unsigned long long)c.s_nfe);
-    atomicAdd(&P.stats[3], (unsigned long long)c.s_nje);
-    atomicAdd(&P.stats[4], (unsigned long long)c.s_nset);
-    atomicAdd(&P.stats[5], (unsigned long long)c.s_nni);
-    atomicAdd(&P.stats[6], (unsigned long long)c.s_netf);
-    atomicAdd(&P.stats[7], (unsigned long long)c.s_ncfn);
-    if (c.s_fail) atomicAdd(P.nfail, c.s_fail);
+    atomicAdd(&P.stats[0], (unsigned long long)ct->s_cells);
+    atomicAdd(&P.stats[1], (unsigned long long)ct->s_nst);
+    atomicAdd(&P.stats[2], (unsigned long long)ct->s_nfe);
+    atomicAdd(&P.stats[3], (unsigned long long)ct->s_nje);
+    atomicAdd(&P.stats[4], (unsigned long long)ct->s_nset);
+    atomicAdd(&P.stats[5], (unsigned long long)ct->s_nni);
+    atomicAdd(&P.stats[6], (unsigned long long)ct->s_netf);
+    atomicAdd(&P.stats[7], (unsigned long long)ct->s_ncfn);
+    if (ct->s_fail) atomicAdd(P.nfail, ct->s_fail);
   }
 }
 
