@@ -382,29 +382,34 @@ __device__ __forceinline__ void rx_eval(const RxRec& rx, const double* sC, const
   }
   const int r0 = rx.reac & 0xFF, r1 = (rx.reac >> 8) & 0xFF, r2 = (rx.reac >> 16) & 0xFF;
   const int p0 = rx.prod & 0xFF, p1 = (rx.prod >> 8) & 0xFF, p2 = (rx.prod >> 16) & 0xFF;
-  double pr = sC[r0];
-  if (nre > 1) pr *= sC[r1];
-  if (nre > 2) pr *= sC[r2];
-  double pp = sC[p0];
-  if (npr > 1) pp *= sC[p1];
-  if (npr > 2) pp *= sC[p2];
+  // The six species slots are read unconditionally and together (an absent slot holds species 0,
+  // whose value is read and not used): one LDS round trip instead of one per present slot.
+  const double cr0 = sC[r0], cr1 = sC[r1], cr2 = sC[r2], cp0 = sC[p0], cp1 = sC[p1], cp2 = sC[p2];
+  const double gr0 = sG[r0], gr1 = sG[r1], gr2 = sG[r2], gp0 = sG[p0], gp1 = sG[p1], gp2 = sG[p2];
+  double pr = cr0;
+  if (nre > 1) pr *= cr1;
+  if (nre > 2) pr *= cr2;
+  double pp = cp0;
+  if (npr > 1) pp *= cp1;
+  if (npr > 2) pp *= cp2;
   double invKc = 0.0, dH = 0.0;
   const int sumnu = npr - nre;
   if (rev) {
-    double dG = sG[p0];
-    if (npr > 1) dG += sG[p1];
-    if (npr > 2) dG += sG[p2];
-    dG -= sG[r0];
-    if (nre > 1) dG -= sG[r1];
-    if (nre > 2) dG -= sG[r2];
+    double dG = gp0;
+    if (npr > 1) dG += gp1;
+    if (npr > 2) dG += gp2;
+    dG -= gr0;
+    if (nre > 1) dG -= gr1;
+    if (nre > 2) dG -= gr2;
     invKc = pr_exp(fma((double)sumnu, lnRTP, dG));
     if (JAC) {
-      dH = sH[p0];
-      if (npr > 1) dH += sH[p1];
-      if (npr > 2) dH += sH[p2];
-      dH -= sH[r0];
-      if (nre > 1) dH -= sH[r1];
-      if (nre > 2) dH -= sH[r2];
+      const double hr0 = sH[r0], hr1 = sH[r1], hr2 = sH[r2], hp0 = sH[p0], hp1 = sH[p1], hp2 = sH[p2];
+      dH = hp0;
+      if (npr > 1) dH += hp1;
+      if (npr > 2) dH += hp2;
+      dH -= hr0;
+      if (nre > 1) dH -= hr1;
+      if (nre > 2) dH -= hr2;
     }
   }
   const double dlnKc = (dH - (double)sumnu) * invT;
@@ -600,10 +605,22 @@ __device__ __forceinline__ void net_rates(const Smem<N, R>& S, int lane, double 
   for (int b = lane; b - lane < nblk; b += G) {
     if (b < nblk) {
       const int e0 = S.blk[b], ne = S.blk[b + 1] - e0;
+      // every table read of the block first (entries past its end re-read its last one), then
+      // the sum in entry order: two LDS round trips per block instead of two per entry
+      int off[PR_GATHER_BLOCK];
+      double nu[PR_GATHER_BLOCK], qv[PR_GATHER_BLOCK];
+#pragma unroll
+      for (int j = 0; j < PR_GATHER_BLOCK; ++j) {
+        const int e = e0 + (j < ne ? j : ne - 1);
+        off[j] = S.incq[e];
+        nu[j] = S.incnu[e];
+      }
+#pragma unroll
+      for (int j = 0; j < PR_GATHER_BLOCK; ++j) qv[j] = *reinterpret_cast<const double*>(qb + off[j]);
       double p = 0.0;
 #pragma unroll
       for (int j = 0; j < PR_GATHER_BLOCK; ++j)
-        if (j < ne) p += S.incnu[e0 + j] * *reinterpret_cast<const double*>(qb + S.incq[e0 + j]);
+        if (j < ne) p += nu[j] * qv[j];
       part[b] = p;
     }
   }
