@@ -151,10 +151,10 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="dodecane_128 / syn53_128: hipbdf.block (with --waves)")
     ap.add_argument("--workload", default="drm19_64",
                     choices=["drm19_64", "dodecane_128", "syn53_128", "amr_mixed", "amr_dodecane", "h2_32"],
-                    help="dodecane_128: plain GPU BDF on n-dodecane (config 3), 128^3 cells at 60 atm in eight "
+                    help="dodecane_128: plain GPU BDF on n-dodecane, 128^3 cells at 60 atm in eight "
                          "262144-cell launches (one step = all eight); syn53_128: the same on the synthetic "
                          "53-species mechanism; amr_mixed / amr_dodecane: learned map + batched fallback over "
-                         "mixed AMR boxes in one hipGraph (drm19 / n-dodecane, config 5)")
+                         "mixed AMR boxes in one hipGraph (drm19 / n-dodecane, about 1.5 M cells on three levels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X; the library has no multi-GPU path (each GPU would run its own "
@@ -378,9 +378,9 @@ def bench_h2(args):
 
 
 def bench_amr(args, mech="drm19"):
-    """Learned map + batched BDF fallback over mixed AMR boxes in one hipGraph (config 5): boxes of
+    """Learned map + batched BDF fallback over mixed AMR boxes in one hipGraph: boxes of
     three levels with edge 16, 32 or 64 drawn uniformly until about 1.5 M cells, per-box dt_react
-    from {1e-6, 2e-6, 4e-6} s (seed 5005 + box). mech "dodecane_lu": config-3 n-dodecane states
+    from {1e-6, 2e-6, 4e-6} s (seed 5005 + box). mech "dodecane_lu": the n-dodecane 60 atm states
     (60 atm, 10 % masked) with the 6 x 512 network; mech "drm19": flame-like drm19 states with 5 %
     Gaussian sources and the 4 x 256 network. One step = restoring every box's state from its
     pristine copy (FC carries over, as PeleLMeX's functC does, so from the second step the cells are
@@ -503,7 +503,7 @@ def bench_amr(args, mech="drm19"):
 
 
 def bench_chunks(args, mech="dodecane_lu"):
-    """Plain GPU BDF on config 3: n-dodecane (dodecane_lu, 53 species, 253 reactions; or the
+    """Plain GPU BDF on n-dodecane (dodecane_lu, 53 species, 253 reactions; or the
     synthetic syn53 of the same size with mech="syn53"), a 128^3 box at 60 atm with 10 % of cells
     masked, dt_react 2e-6 s, as eight 64^3 chunks of 262144 cells (one react launch each). One step
     = the eight launches over the whole box, each chunk restored from a pristine device copy first.

@@ -1,6 +1,6 @@
 """The reactor on n-dodecane (dodecane_lu: 53 species, 253 reactions; 54-component state, 54 x 54
-Jacobian and LU in the cell's global slot at 64 lanes), on config-3 states (mixture fraction
-Z ~ Beta(0.5, 2), T 700-2400 K, 60 atm, 10 % of cells masked), against the CPU oracle bit for bit:
+Jacobian and LU in the cell's global slot at 64 lanes), on states of mixture fraction
+Z ~ Beta(0.5, 2) and T 700-2400 K at 60 atm with 10 % of cells masked, against the CPU oracle bit for bit:
 right-hand sides and Jacobians, whole react calls for both builds of the kernel (2 and 3 waves per
 SIMD), the failure / clipping rules, and a batch of boxes against single boxes."""
 import numpy as np
