@@ -93,6 +93,16 @@ def test_h2_air_parity(gpu, lanes):
     assert np.all(out["FC"] > 0)
 
 
+def test_h2_air_full_box_bits(gpu):
+    """The whole 32^3 H2/air box (32768 cells) at max_order 5, every cell bit for bit."""
+    full = W.h2_air_box()
+    out = _gpu(full, KEYS, gpu)
+    assert out["nfail"] == 0 and np.all(out["status"] == 0)
+    ref = _oracle(full, max_order=5)
+    for k in ("rY", "T", "rH", "FC"):
+        _assert_bits(k, out[k], ref[k])
+
+
 @pytest.mark.parametrize("lanes,waves", [(32, 0), (32, 3), (16, 0), (64, 0)])
 def test_ch4_flame_parity(gpu, lanes, waves):
     full = W.ch4_air_flame_box()
