@@ -142,6 +142,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true", help="measure the plain GPU BDF only")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="default workload only: skip the other workloads measured after it")
     ap.add_argument("--surrogate", default=None, help="trained network (.npz); default: the packaged one")
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -161,13 +163,13 @@ def main():
               "boxes independently).", file=sys.stderr)
         sys.exit(2)
     if args.workload in ("dodecane_128", "syn53_128"):
-        bench_chunks(args, "dodecane_lu" if args.workload == "dodecane_128" else "syn53")
+        print(json.dumps(bench_chunks(args, "dodecane_lu" if args.workload == "dodecane_128" else "syn53")), flush=True)
         return
     if args.workload in ("amr_mixed", "amr_dodecane"):
-        bench_amr(args, "dodecane_lu" if args.workload == "amr_dodecane" else "drm19")
+        print(json.dumps(bench_amr(args, "dodecane_lu" if args.workload == "amr_dodecane" else "drm19")), flush=True)
         return
     if args.workload == "h2_32":
-        bench_h2(args)
+        print(json.dumps(bench_h2(args)), flush=True)
         return
 
     import torch
@@ -315,7 +317,50 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"], ref = cpu_baseline(box, min(args.cpu_sample, ncells), max_order=args.max_order)
         line["parity"] = gpu_vs_oracle(bdf_state, ref)
+    if not args.no_extra:
+        torch.cuda.empty_cache()
+        line["other_workloads"] = other_workloads(args)
     print(json.dumps(line), flush=True)
+
+
+def other_workloads(args):
+    """The other workloads, measured after the headline one in the same run (their own metrics;
+    not part of `value`): the H2/air 32^3 box, the n-dodecane 128^3 box at 60 atm in eight
+    launches, and the learned map + batched fallback over mixed AMR boxes for drm19 and
+    n-dodecane. One timed step each after one warm-up (three for the small H2 box), parity on a
+    CPU-oracle sample where the run has one. A workload that fails is reported as such."""
+    import copy
+    import torch
+    runs = [("h2_32", lambda a: bench_h2(a), 3),
+            ("dodecane_128", lambda a: bench_chunks(a, "dodecane_lu"), 1),
+            ("amr_mixed", lambda a: bench_amr(a, "drm19"), 1),
+            ("amr_dodecane", lambda a: bench_amr(a, "dodecane_lu"), 1)]
+    out = {}
+    for name, fn, steps in runs:
+        a = copy.copy(args)
+        a.workload, a.steps, a.warmup, a.waves, a.block, a.copy_probe_mb = name, steps, 1, 0, 0, 0
+        a.cpu_sample = min(args.cpu_sample, 4096)
+        print(f"other workload {name}", file=sys.stderr, flush=True)
+        try:
+            ln = fn(a)
+        except Exception as e:  # reported, not raised: the headline line stands on its own
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+            continue
+        finally:
+            torch.cuda.empty_cache()
+        keep = {k: ln[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "warmup", "bdf_value",
+                                   "failed_cells", "parity") if k in ln}
+        keep["workload"] = ln["config"].get("workload")
+        keep["roofline_frac"] = ln["roofline"]["frac"]
+        if "cpu_baseline" in ln:
+            keep["cpu_baseline"] = {k: ln["cpu_baseline"][k] for k in ("value", "unit", "cores", "sample")}
+        if ln.get("surrogate"):
+            keep["surrogate"] = {k: ln["surrogate"].get(k) for k in ("value", "speedup_vs_bdf", "accepted",
+                                                                      "audit_max_dT_rel")}
+        if "integrator" in ln and "failed_cells_per_step" in ln["integrator"]:
+            keep["failed_cells_per_step"] = ln["integrator"]["failed_cells_per_step"]
+        out[name] = keep
+    return out
 
 
 def bench_h2(args):
@@ -374,7 +419,7 @@ def bench_h2(args):
         line["cpu_baseline"], ref = cpu_baseline(box, n, what="the same 32^3 box (every cell)",
                                                  max_order=args.max_order)
         line["parity"] = gpu_vs_oracle(t, ref)
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def bench_amr(args, mech="drm19"):
@@ -499,7 +544,7 @@ def bench_amr(args, mech="drm19"):
                          counts["N"], counts["R"], lanes),
                      "flops_per_step": F},
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def bench_chunks(args, mech="dodecane_lu"):
@@ -608,7 +653,7 @@ def bench_chunks(args, mech="dodecane_lu"):
                                                  what="the first 64^3 chunk (masked cells count as cells)")
         line["parity"] = gpu_vs_oracle(chunks[0][1], ref)   # chunk 0's tensors hold its last result
         line["parity"]["box"] = "the first 64^3 chunk, the CPU sample's cells"
-    print(json.dumps(line), flush=True)
+    return line
 
 
 if __name__ == "__main__":
