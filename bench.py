@@ -331,7 +331,8 @@ def other_workloads(args):
     CPU-oracle sample where the run has one. A workload that fails is reported as such."""
     import copy
     import torch
-    runs = [("h2_32", lambda a: bench_h2(a), 3),
+    runs = [("box_kernels_drm19_64", lambda a: bench_box_kernels(a), 20),
+            ("h2_32", lambda a: bench_h2(a), 3),
             ("dodecane_128", lambda a: bench_chunks(a, "dodecane_lu"), 1),
             ("amr_mixed", lambda a: bench_amr(a, "drm19"), 1),
             ("amr_dodecane", lambda a: bench_amr(a, "dodecane_lu"), 1)]
@@ -348,6 +349,9 @@ def other_workloads(args):
             continue
         finally:
             torch.cuda.empty_cache()
+        if name.startswith("box_kernels"):
+            out[name] = ln
+            continue
         keep = {k: ln[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "warmup", "bdf_value",
                                    "failed_cells", "parity") if k in ln}
         keep["workload"] = ln["config"].get("workload")
@@ -360,6 +364,58 @@ def other_workloads(args):
         if "integrator" in ln and "failed_cells_per_step" in ln["integrator"]:
             keep["failed_cells_per_step"] = ln["integrator"]["failed_cells_per_step"]
         out[name] = keep
+    return out
+
+
+def bench_box_kernels(args):
+    """The box kernels around the chemistry on the 64^3 drm19 flame box (MKS arrays as PeleLMeX
+    holds them): reactionRateRhoY, setTemperature (from the stored T), floorSpecies, I_R and
+    getHeatRelease. Each is timed over `steps` calls after two warm-up calls with HIP events on
+    the launch stream, so a call's time includes its launch and the entry point's own stream
+    synchronization; bytes are the algorithmic state traffic per cell (doubles read + written)
+    and the rate is set against the 8 TB/s HBM peak (the first two evaluate the rates / the
+    species thermo per cell and are bound by that, the others by HBM)."""
+    import torch
+    from pelelmex_amd import roofline, workloads
+    from pelelmex_amd.reactor import Reactor, box_tensors
+    box = workloads.ch4_air_flame_box()
+    n = box["T"].size
+    N = box["rY"].shape[0]
+    dev = torch.device("cuda:0")
+    t = box_tensors(box, dev)
+    rY = (t["rY"] * 1e3).contiguous()
+    rH = (t["rH"] * 0.1).contiguous()
+    T = t["T"].clone()
+    rho = rY.sum(0).contiguous()
+    r = Reactor.create("ReactorCvode", "drm19", {"ode.rtol": 1e-10, "ode.atol": 1e-10})
+    r.init(2, 1)
+    rYdot = torch.empty_like(rY)
+    HR = torch.empty_like(T)
+    IR = torch.empty_like(rY)
+    F = torch.zeros_like(rY)
+    rY_old = rY.clone()
+    ops = [("reactionRateRhoY", lambda: r.reaction_rate_rhoY(rY, rH, T, rYdot=rYdot), (2 * N + 2) * 8),
+           ("setTemperature", lambda: r.set_temperature(rho, rY, rH, T), (N + 4) * 8),
+           ("floorSpecies", lambda: r.floor_species(rY, rho, rH, T), (2 * N + 3) * 8),
+           ("I_R", lambda: r.reaction_source(rY_old, rY, F, 1e-6, IR=IR), 4 * N * 8),
+           ("getHeatRelease", lambda: r.heat_release(T, rYdot, HR=HR), (N + 2) * 8)]
+    out = {"workload": "drm19 CH4/air flame-like box 64^3 = 262144 cells, MKS state arrays",
+           "calls_timed": args.steps}
+    for name, fn, bpc in ops:
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        gbs = bpc * n / (ms / 1e3) / 1e9
+        out[name] = {"ms_per_call": ms, "cells_per_s": n / (ms / 1e3), "bytes_per_cell": bpc, "GBs": gbs,
+                     "frac_of_hbm_peak": gbs / roofline.MI355X_HBM_GBS}
+    r.close()
     return out
 
 

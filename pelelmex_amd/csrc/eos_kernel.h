@@ -1,14 +1,17 @@
-// Equation-of-state box kernels around the chemistry advance, MKS Array4 views, one group of G
-// lanes per cell (species rows as in the reactor):
+// Equation-of-state box kernels around the chemistry advance, MKS Array4 views, one thread per
+// cell (cells fastest, so every species row of the box is read and written coalesced; the
+// species records are staged in LDS and read by all lanes at once):
 //  * setTemperature  (Source/PeleLMeX_Eos.cpp:218-240 -> getTfromHY, Source/PeleLMeX_K.H:700-723):
 //    rhoinv = 1 / rho, Y = rhoY * rhoinv, h = rhoH * rhoinv * 1e4, T = HY2T(h, Y) from the stored
-//    T - the reactor's own hy2t, so the temperature is the one its right-hand side computes;
+//    T - the reactor's own iteration and sums (hy2t), so the temperature is the one its
+//    right-hand side computes;
 //  * floorSpecies    (Source/PeleLMeX_Utils.cpp:649-690, fabMinMax Source/PeleLMeX_K.H:894-907):
 //    rhoY_k = max(0, min(DBL_MAX, rhoY_k)) with std::min / std::max semantics, rho = sum_k rhoY_k
 //    and h = TY2H(T, Y) = sum_k Y_k h_k(T) in species order, rhoH = h * 1e-4 * rho;
 //  * getHeatRelease  (Source/PeleLMeX_Reactions.cpp:477-506, getHGivenT Source/PeleLMeX_K.H:598-615):
 //    HR = 0 - sum_k (h_k(T) * 1e-4) * I_R,k in species order.
-// h_k(T) = (h_k / RT) (R / W_k) T in erg / g, the reactor's species enthalpy.
+// h_k(T) = (h_k / RT) (R / W_k) T in erg / g, the reactor's species enthalpy. HBM-bound: a cell
+// moves (N + 4), (2 N + 3) and (N + 2) doubles.
 #pragma once
 
 #include <cfloat>
@@ -27,39 +30,68 @@ struct EosParams {
   int blob_bytes, neff, ninc;
 };
 
-// Sum of the per-row values over species 0..N-1 in species order, starting from 0.0 (SUB:
-// 0.0 - v_0 - v_1 - ...), through the group's rate area in LDS; returned to every lane.
-template <int N, int R, int G, bool SUB>
-__device__ __forceinline__ double seq_sum(const Smem<N, R>& S, int lane, const double (&v)[Cell<N, R, G>::RPL]) {
-  group_sync();  // earlier readers of the area are done
+// hy2t of the reactor (bdf_kernel.h) by one thread: the same Newton iteration, with the mixture
+// enthalpy and heat capacity summed over the canonical pairwise tree of the reactor's component
+// sums (64 zero-padded slots: pairs, pairs of pairs, ...). The tree is evaluated as the slots are
+// formed, keeping one pending left subtree per level, so only six partial sums per quantity are
+// live; Y_k = rhoY_k / rho is re-read from the cell's row at each iteration.
+template <int N>
+__device__ __forceinline__ double hy2t_cell(const SpRec* sp, const double* rY, long ns, double rhoinv, double h,
+                                            double T) {
+  static_assert(N <= 64, "at most 64 components");
+  for (int it = 0; it < 20; ++it) {
+    const double invT = 1.0 / T;
+    double sh[6], sc[6], hm = 0.0, cpm = 0.0;
+#pragma unroll 1
+    for (int s = 0; s < 64; ++s) {
+      double a = 0.0, b = 0.0;
+      if (s < N) {
+        const double Y = rY[(long)s * ns] * rhoinv;
+        double cpR, hRT;
+        nasa_hcp(sp[s], T, invT, cpR, hRT);
+        a = ((Y * hRT) * sp[s].RW) * T;
+        b = (Y * cpR) * sp[s].RW;
+      }
+      bool placed = false;
 #pragma unroll
-  for (int s = 0; s < Cell<N, R, G>::RPL; ++s) {
-    const int i = lane + G * s;
-    if (i < N) S.q[i] = v[s];
+      for (int l = 0; l < 6; ++l) {
+        if (!placed) {
+          if ((s >> l) & 1) {  // the right subtree at level l: join its pending left sibling
+            a = sh[l] + a;
+            b = sc[l] + b;
+          } else {  // a left subtree: pending until its sibling is complete
+            sh[l] = a;
+            sc[l] = b;
+            placed = true;
+          }
+        }
+      }
+      if (s == 63) {  // every level has joined: the totals
+        hm = a;
+        cpm = b;
+      }
+    }
+    const double dT = (h - hm) / cpm;
+    T = T + dT;
+    if (fabs(dT) <= 1e-10 * T) break;
   }
-  group_sync();
-  double a = 0.0;
-  for (int k = 0; k < N; ++k) a = SUB ? a - S.q[k] : a + S.q[k];
-  return a;
+  return T;
 }
 
-template <int N, int R, int G, int MODE>
-__global__ void __launch_bounds__(BLOCK) eos_kernel(EosParams P) {
-  constexpr int RPL = Cell<N, R, G>::RPL;
+template <int N, int R, int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) eos_cell_kernel(EosParams P) {
   extern __shared__ __align__(16) unsigned char smem[];
   {
-    const uint4* srcb = reinterpret_cast<const uint4*>(P.blob);
+    // the species records only: the blob's first section
+    const BlobLayout BL = blob_layout(N, R, P.neff, P.ninc);
+    const uint4* srcb = reinterpret_cast<const uint4*>(P.blob + BL.off_sp);
     uint4* dst = reinterpret_cast<uint4*>(smem);
-    for (int b = threadIdx.x; b < P.blob_bytes / 16; b += blockDim.x) dst[b] = srcb[b];
+    for (int b = threadIdx.x; b < (int)(N * sizeof(SpRec)) / 16; b += blockDim.x) dst[b] = srcb[b];
   }
   __syncthreads();
-  const BlobLayout BL = blob_layout(N, R, P.neff, P.ninc);
-  Smem<N, R> S;
-  const int lane = threadIdx.x % G;
-  const int grp = threadIdx.x / G;
-  setup_views<N, R, G>(S, smem, BL, grp, nullptr);
-  const long stride = (long)gridDim.x * (blockDim.x / G);
-  for (long c = (long)blockIdx.x * (blockDim.x / G) + grp; c < P.ncells; c += stride) {  // whole groups
+  const SpRec* sp = reinterpret_cast<const SpRec*>(smem);
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < P.ncells; c += stride) {
     const unsigned u = (unsigned)c;
     const int i = P.lo0 + (int)(u % (unsigned)P.nx);
     const int j = P.lo1 + (int)((u / (unsigned)P.nx) % (unsigned)P.ny);
@@ -67,68 +99,44 @@ __global__ void __launch_bounds__(BLOCK) eos_kernel(EosParams P) {
     if constexpr (MODE == EOS_SET_T) {
       const double rhoinv = 1.0 / P.rho.p[aoff(P.rho, i, j, k)];
       const long offY = aoff(P.rY, i, j, k), oT = aoff(P.T, i, j, k);
-      double Y[RPL];
-#pragma unroll
-      for (int s = 0; s < RPL; ++s) {
-        const int ii = lane + G * s;
-        Y[s] = (ii < N) ? P.rY.p[offY + (long)ii * P.rY.ns] * rhoinv : 0.0;
-      }
-      const double T = hy2t<N, R, G>(S, lane, Y, (P.rH.p[aoff(P.rH, i, j, k)] * rhoinv) * 1.0e4, P.T.p[oT]);
-      if (lane == 0) P.T.p[oT] = T;
+      P.T.p[oT] = hy2t_cell<N>(sp, P.rY.p + offY, P.rY.ns, rhoinv, (P.rH.p[aoff(P.rH, i, j, k)] * rhoinv) * 1.0e4,
+                               P.T.p[oT]);
     } else if constexpr (MODE == EOS_FLOOR) {
       const long offY = aoff(P.rY, i, j, k);
-      double v[RPL];
-#pragma unroll
-      for (int s = 0; s < RPL; ++s) {
-        const int ii = lane + G * s;
-        v[s] = 0.0;
-        if (ii < N) {
-          double x = P.rY.p[offY + (long)ii * P.rY.ns];
-          x = (x < DBL_MAX) ? x : DBL_MAX;  // min(DBL_MAX, x): NaN becomes DBL_MAX
-          x = (0.0 < x) ? x : 0.0;          // max(0, x): negatives and -0 become +0
-          v[s] = x;
-          P.rY.p[offY + (long)ii * P.rY.ns] = x;
-        }
+      double rho = 0.0;
+#pragma unroll 4
+      for (int s = 0; s < N; ++s) {
+        double x = P.rY.p[offY + (long)s * P.rY.ns];
+        x = (x < DBL_MAX) ? x : DBL_MAX;  // min(DBL_MAX, x): NaN becomes DBL_MAX
+        x = (0.0 < x) ? x : 0.0;          // max(0, x): negatives and -0 become +0
+        P.rY.p[offY + (long)s * P.rY.ns] = x;
+        rho += x;
       }
-      const double rho = seq_sum<N, R, G, false>(S, lane, v);
       const double rhoinv = 1.0 / rho;
       const double T = P.T.p[aoff(P.T, i, j, k)];
       const double invT = 1.0 / T;
-      double t[RPL];
-#pragma unroll
-      for (int s = 0; s < RPL; ++s) {
-        const int ii = lane + G * s;
-        t[s] = 0.0;
-        if (ii < N) {
-          double cpR, hRT;
-          nasa_hcp(S.sp[ii], T, invT, cpR, hRT);
-          t[s] = ((v[s] * rhoinv * hRT) * S.sp[ii].RW) * T;
-        }
+      double h = 0.0;
+#pragma unroll 4
+      for (int s = 0; s < N; ++s) {  // the floored values just written, re-read from the row
+        double cpR, hRT;
+        nasa_hcp(sp[s], T, invT, cpR, hRT);
+        h += ((P.rY.p[offY + (long)s * P.rY.ns] * rhoinv * hRT) * sp[s].RW) * T;
       }
-      const double h = seq_sum<N, R, G, false>(S, lane, t);
-      if (lane == 0) {
-        P.rho.p[aoff(P.rho, i, j, k)] = rho;
-        P.rH.p[aoff(P.rH, i, j, k)] = (h * 1.0e-4) * rho;
-      }
+      P.rho.p[aoff(P.rho, i, j, k)] = rho;
+      P.rH.p[aoff(P.rH, i, j, k)] = (h * 1.0e-4) * rho;
     } else {
       const double T = P.T.p[aoff(P.T, i, j, k)];
       const double invT = 1.0 / T;
       const long offR = aoff(P.IR, i, j, k);
-      double p[RPL];
-#pragma unroll
-      for (int s = 0; s < RPL; ++s) {
-        const int ii = lane + G * s;
-        p[s] = 0.0;
-        if (ii < N) {
-          double cpR, hRT;
-          nasa_hcp(S.sp[ii], T, invT, cpR, hRT);
-          p[s] = (((hRT * S.sp[ii].RW) * T) * 0.0001) * P.IR.p[offR + (long)ii * P.IR.ns];
-        }
+      double hr = 0.0;
+#pragma unroll 4
+      for (int s = 0; s < N; ++s) {
+        double cpR, hRT;
+        nasa_hcp(sp[s], T, invT, cpR, hRT);
+        hr -= (((hRT * sp[s].RW) * T) * 0.0001) * P.IR.p[offR + (long)s * P.IR.ns];
       }
-      const double hr = seq_sum<N, R, G, true>(S, lane, p);
-      if (lane == 0) P.HR.p[aoff(P.HR, i, j, k)] = hr;
+      P.HR.p[aoff(P.HR, i, j, k)] = hr;
     }
-    group_sync();  // the next cell's stores into the rate area wait for this cell's readers
   }
 }
 

@@ -106,11 +106,11 @@ hipError_t prep_t(const prk::PrepParams& P, int blocks, int block, size_t shmem,
 template <int N, int R, int G>
 hipError_t eos_t(int mode, const prk::EosParams& P, int blocks, int block, size_t shmem, hipStream_t st) {
   if (mode == prk::EOS_SET_T)
-    hipLaunchKernelGGL((prk::eos_kernel<N, R, G, prk::EOS_SET_T>), dim3(blocks), dim3(block), shmem, st, P);
+    hipLaunchKernelGGL((prk::eos_cell_kernel<N, R, prk::EOS_SET_T>), dim3(blocks), dim3(block), shmem, st, P);
   else if (mode == prk::EOS_FLOOR)
-    hipLaunchKernelGGL((prk::eos_kernel<N, R, G, prk::EOS_FLOOR>), dim3(blocks), dim3(block), shmem, st, P);
+    hipLaunchKernelGGL((prk::eos_cell_kernel<N, R, prk::EOS_FLOOR>), dim3(blocks), dim3(block), shmem, st, P);
   else
-    hipLaunchKernelGGL((prk::eos_kernel<N, R, G, prk::EOS_HEAT_RELEASE>), dim3(blocks), dim3(block), shmem, st, P);
+    hipLaunchKernelGGL((prk::eos_cell_kernel<N, R, prk::EOS_HEAT_RELEASE>), dim3(blocks), dim3(block), shmem, st, P);
   return hipGetLastError();
 }
 
@@ -1620,7 +1620,7 @@ int pr_reaction_rate_box(pr_handle* h, const int lo[3], const int hi[3], const p
   return 0;
 }
 
-// Launches one of the EOS box kernels (one group per cell, grid-stride) and synchronizes.
+// Launches one of the EOS box kernels (one thread per cell, grid-stride) and synchronizes.
 static int eos_launch(pr_handle* h, int mode, const int lo[3], const int hi[3], prk::EosParams& P, void* stream) {
   std::lock_guard<std::recursive_mutex> lock(h->mu);
   const long nx = (long)hi[0] - lo[0] + 1, ny = (long)hi[1] - lo[1] + 1, nz = (long)hi[2] - lo[2] + 1;
@@ -1638,12 +1638,11 @@ static int eos_launch(pr_handle* h, int mode, const int lo[3], const int hi[3], 
   P.blob_bytes = h->blob_bytes;
   P.neff = h->mech->neff;
   P.ninc = h->mech->ninc;
-  const int G = h->var->G;
-  const int block = pick_block(h);
-  const size_t shmem = (size_t)h->blob_bytes + (size_t)(block / G) * h->var->scratch;
+  const int block = 256;
+  const size_t shmem = (size_t)h->mech->nsp * sizeof(prk::SpRec);  // the species records
   int dev_cus = 0;
   HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, h->device));
-  const long want = (ncells + (block / G) - 1) / (block / G);
+  const long want = (ncells + block - 1) / block;
   const long cap = 8L * dev_cus;
   const int blocks = (int)(want < cap ? want : cap);
   hipStream_t st = (hipStream_t)stream;
