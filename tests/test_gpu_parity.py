@@ -346,8 +346,8 @@ def test_block_size_changes_nothing_but_speed(gpu):
 def test_array4_views_inside_ghosted_fabs(gpu):
     """Drop-in layout: the box [lo, hi] (lo != 0) is a view into FABs with two ghost cells per
     side (AMReX Array4: p at the FAB's begin corner, FAB strides, component stride = FAB cells).
-    react and reactionRateRhoY on the views give the bits of contiguous runs on the same cells,
-    and the ghost cells (NaN) are neither read into results nor written."""
+    react, reactionRateRhoY and the EOS box kernels on the views give the bits of contiguous runs
+    on the same cells, and the ghost cells (NaN) are neither read into results nor written."""
     import ctypes as C
     import torch
     from pelelmex_amd import reactor as RM
@@ -404,6 +404,36 @@ def test_array4_views_inside_ghosted_fabs(gpu):
                                 dev(box["T"], nz, ny, nx), mask=dev(box["mask"], nz, ny, nx))
     torch.cuda.synchronize()
     _assert_bits("rYdot", out.cpu()[inner].numpy(), want.cpu().numpy())
+    # the EOS box kernels on views of the same FABs (one thread per cell, strided rows), against
+    # contiguous calls; floorSpecies gets negative and NaN entries, and no ghost is written
+    rYm = box["rY"] * 1e3
+    rYm[3, ::5] = -1e-9
+    rYm[7, ::11] = np.nan
+    rho = np.nansum(np.abs(rYm), axis=0)
+    vT, vrho, vY, vH = fab(box["T"] * 1.01, 1), fab(rho, 1), fab(rYm, N), fab(box["rH"] * 0.1, 1)
+    vIR, vHR = fab(np.sin(np.arange(N * box["T"].size)).reshape(N, -1), N), fab(np.zeros(box["T"].size), 1)
+    rc = L.pr_set_temperature_box(r._h, clo, chi, C.byref(a4(vrho, 1)), C.byref(a4(fab(np.abs(box["rY"]) * 1e3, N), N)),
+                                  C.byref(a4(vH, 1)), C.byref(a4(vT, 1)), st)
+    assert rc == 0, L.pr_last_error()
+    cT = dev(box["T"] * 1.01, nz, ny, nx)
+    r.set_temperature(dev(rho, nz, ny, nx), dev(np.abs(box["rY"]) * 1e3, N, nz, ny, nx), dev(box["rH"] * 0.1, nz, ny, nx),
+                      cT)
+    rc = L.pr_floor_species_box(r._h, clo, chi, C.byref(a4(vY, N)), C.byref(a4(vrho, 1)), C.byref(a4(vH, 1)),
+                                C.byref(a4(vT, 1)), st)
+    assert rc == 0, L.pr_last_error()
+    cY, crho, cH = dev(rYm, N, nz, ny, nx), dev(rho, nz, ny, nx), dev(box["rH"] * 0.1, nz, ny, nx)
+    r.floor_species(cY, crho, cH, cT)
+    rc = L.pr_heat_release_box(r._h, clo, chi, C.byref(a4(vT, 1)), C.byref(a4(vIR, N)), C.byref(a4(vHR, 1)), st)
+    assert rc == 0, L.pr_last_error()
+    cHR = r.heat_release(cT, dev(np.sin(np.arange(N * box["T"].size)).reshape(N, -1), N, nz, ny, nx))
+    torch.cuda.synchronize()
+    for name, v, c, ncomp in (("T", vT, cT, 1), ("rhoY", vY, cY, N), ("rho", vrho, crho, 1), ("rhoH", vH, cH, 1),
+                              ("HR", vHR, cHR, 1)):
+        got = v.cpu()
+        _assert_bits(name, got[inner].reshape(ncomp, -1).numpy().squeeze(), c.cpu().reshape(ncomp, -1).numpy().squeeze())
+        ghost = got.clone()
+        ghost[inner] = 0.0
+        assert torch.isnan(ghost).sum() == ghost.numel() - got[inner].numel(), f"{name}: a ghost cell was written"
     r.close()
 
 
