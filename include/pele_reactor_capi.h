@@ -144,7 +144,8 @@ typedef struct {
    surrogate.enable = 1) every box's cells go through it first - features per box, one network
    pass over all boxes, the FP64 trust check per box with its dt - and the persistent launch
    integrates the rejected cells of all boxes; results are those of pr_react_box with the same
-   map, box by box. Returns the number of failed cells (>= 0) or a negative error code. */
+   map, box by box. Returns the number of failed cells (>= 0) or a negative error code. nbox = 0
+   (boxes may be NULL: a level with no box on this rank) integrates nothing and returns 0. */
 int pr_react_batch(pr_handle* h, int nbox, const pr_box* boxes, void* stream);
 /* pr_react_batch in two halves, for callers that capture the work in a hipGraph: _enqueue only
    enqueues on `stream` (a copy of the box table from the handle's pinned staging memory, the

@@ -1351,8 +1351,22 @@ int pr_react_box_noalloc(pr_handle* h, const int lo[3], const int hi[3], const p
 }
 
 int pr_react_batch_enqueue(pr_handle* h, int nbox, const pr_box* boxes, void* stream) {
-  if (!h || nbox < 1 || !boxes) return fail(PR_ERR_ARG, "bad argument");
+  if (!h || nbox < 0 || (nbox > 0 && !boxes)) return fail(PR_ERR_ARG, "bad argument");
   std::lock_guard<std::recursive_mutex> lock(h->mu);
+  if (nbox == 0) {
+    // A level with no box on this rank (PeleLMeX's MFIter loop then runs zero times): nothing to
+    // integrate. The counters are cleared on the stream so that pr_collect reports 0 failed
+    // cells and empty statistics; a kernel, not a memset, so the call can be captured.
+    HIPCHK(hipSetDevice(h->device));
+    auto* base = static_cast<unsigned char*>(h->d_counters);
+    hipLaunchKernelGGL(prk::zero_counters, dim3(1), dim3(64), 0, (hipStream_t)stream,
+                       reinterpret_cast<unsigned int*>(base), reinterpret_cast<unsigned long long*>(base + 64),
+                       reinterpret_cast<unsigned int*>(base + 4));
+    HIPCHK(hipGetLastError());
+    h->batch_surr = 0;
+    h->batch_acap = 0;
+    return 0;
+  }
   const int N = h->mech->nsp;
   std::vector<prk::BoxDesc> descs((size_t)nbox);
   std::vector<long> starts((size_t)nbox + 1, 0L);

@@ -150,6 +150,69 @@ def test_mask_skips_cells(gpu):
     _assert_bits("T", out["T"], ref["T"])
 
 
+@pytest.mark.parametrize("lanes", [32, 64])
+def test_single_cell_ragged_and_fully_masked_boxes(gpu, lanes):
+    """Edge boxes of the persistent launch, bit for bit against the oracle: one cell (far fewer
+    cells than resident groups), a ragged 3 x 5 x 7 box (an odd count: at 32 lanes the last
+    wavefront's second cell slot finds the queue empty while the first integrates), and a fully
+    masked box (no active cell; state unchanged, FC = 0, rho h still advanced by its source)."""
+    full = W.ch4_air_flame_box(nx=16, ny=8, nz=4, seed=31)
+    keys = dict(KEYS, **{"cvode.max_order": 4, "hipbdf.lanes_per_cell": lanes})
+    one = W.subsample(full, np.array([295]))          # a front-band cell (i = 7, x = 0.47)
+    rag = W.subsample(full, np.random.default_rng(4).choice(full["T"].size, 105, replace=False))
+    rag["dims"] = (3, 5, 7)
+    masked = dict(rag, mask=np.full(105, -1, dtype=np.int32))
+    for name, box in (("one", one), ("ragged", rag), ("masked", masked)):
+        out = _gpu(box, keys, gpu)
+        ref = _oracle(box, max_order=4)
+        assert out["nfail"] == 0 and np.all(out["status"] == 0), name
+        for k in ("rY", "T", "rH", "FC"):
+            _assert_bits(f"{name} {k}", out[k], ref[k])
+        if name == "masked":
+            np.testing.assert_array_equal(out["rY"], box["rY"])
+            np.testing.assert_array_equal(out["T"], box["T"])
+            assert np.all(out["FC"] == 0) and out["stats"]["cells"] == 0
+        else:
+            assert np.all(out["FC"] > 0) and out["stats"]["cells"] == box["T"].size
+
+
+def test_argument_errors_and_empty_batch(gpu):
+    """Host-side argument checks of pr_react_box / pr_react_batch (no launch happens): an empty box
+    (hi < lo) and a dt_react that is zero, negative or not finite are errors with a message; an
+    empty box list (a level with no box on this rank) integrates nothing and returns 0, and
+    pr_collect after it reports 0 failed cells and empty statistics."""
+    import ctypes as C
+    import torch
+    from pelelmex_amd import reactor as R
+    from pelelmex_amd.reactor import Reactor, ReactorError, box_tensors
+    box = W.ch4_air_flame_box(nx=4, ny=2, nz=2)
+    r = Reactor.create("ReactorCvode", "drm19", KEYS)
+    r.init(2, 1)
+    t = box_tensors(box, gpu)
+    for dt in (0.0, -1e-6, float("nan"), float("inf")):
+        with pytest.raises(ReactorError, match="dt_react must be positive"):
+            r.react(t["rY"], t["rYsrc"], t["T"], t["rH"], t["rHsrc"], t["FC"], t["mask"], dt)
+    a = [R._array4(t["rY"], r.nsp), R._array4(t["rYsrc"], r.nsp)] + \
+        [R._array4(t[k], 1) for k in ("T", "rH", "rHsrc", "FC")]
+    lo = (C.c_int * 3)(0, 0, 0)
+    hi = (C.c_int * 3)(3, -1, 1)
+    dt = C.c_double(box["dt"])
+    tm = C.c_double(0.0)
+    rc = R.lib().pr_react_box(r._h, lo, hi, *[C.byref(x) for x in a], None, C.byref(dt), C.byref(tm),
+                              C.c_void_p(torch.cuda.current_stream().cuda_stream), None)
+    assert rc == -1 and "empty box" in R.lib().pr_last_error().decode()
+    # one real batch first, so that the counters hold something the empty batch must clear
+    b = {k: v.clone() for k, v in t.items()}
+    b["dt"] = box["dt"]
+    assert r.react_batch([b]) == 0 and r.stats()["cells"] == box["T"].size
+    assert r.react_batch([]) == 0
+    st = r.stats()
+    assert st["cells"] == 0 and st["nst"] == 0 and st["failed"] == 0
+    assert r.react_batch([], enqueue_only=True) == 0 and r.collect() == 0
+    torch.cuda.synchronize()
+    r.close()
+
+
 @pytest.mark.parametrize("mech_box,lanes", [("h2", 16), ("h2", 8), ("ch4", 32), ("ch4", 16), ("ch4", 64)])
 def test_rhs_and_jacobian_bits(gpu, mech_box, lanes):
     """Batched right-hand sides and Jacobians on 256 states of each workload, bit for bit, for
